@@ -1,0 +1,1232 @@
+// gctf_engine.hip -- host engine and C-ABI of libgctf_hip.so.
+//
+// The engine owns per-shape plans (FFT tables, tile grid, ring tables),
+// device buffers sized for a batch of micrographs, and the per-micrograph
+// search state.  A batch runs as: [rescale -> tile spectrum] per chunk of
+// images, then background -> gate -> low-pass -> radial average -> 1D
+// search -> 2D search -> spectral CC for the whole batch.  No step waits on
+// the host; the 2D search's control flow runs on the device, so a batch is
+// captured into one HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/gctf_hip.h"
+#include "gctf_internal.hpp"
+#include "k_spectrum.hpp"
+#include "k_search.hpp"
+
+using namespace gctf;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// small host helpers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+	void* p = nullptr;
+	size_t bytes = 0;
+	DevBuf() = default;
+	DevBuf(const DevBuf&) = delete;
+	DevBuf& operator=(const DevBuf&) = delete;
+	~DevBuf() { release(); }
+	void release()
+	{
+		if (p) (void)hipFree(p);
+		p = nullptr;
+		bytes = 0;
+	}
+	// grows to at least b bytes; returns true when the address changed
+	hipError_t ensure(size_t b, bool* moved)
+	{
+		if (b <= bytes) return hipSuccess;
+		release();
+		hipError_t e = hipMalloc(&p, b);
+		if (e != hipSuccess) { p = nullptr; return e; }
+		bytes = b;
+		if (moved) *moved = true;
+		return hipSuccess;
+	}
+	template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+int next_pow2(int n)
+{
+	int m = 1;
+	while (m < n) m <<= 1;
+	return m;
+}
+
+void host_fft(std::vector<std::complex<double>>& a, bool inv)
+{
+	const int n = (int)a.size();
+	for (int i = 1, j = 0; i < n; i++) {
+		int bit = n >> 1;
+		for (; j & bit; bit >>= 1) j ^= bit;
+		j ^= bit;
+		if (i < j) std::swap(a[i], a[j]);
+	}
+	for (int len = 2; len <= n; len <<= 1) {
+		for (int k = 0; k < len / 2; k++) {
+			double ang = (inv ? 2.0 : -2.0) * M_PI * k / len;
+			std::complex<double> w(cos(ang), sin(ang));
+			for (int i = k; i < n; i += len) {
+				std::complex<double> u = a[i], v = a[i + len / 2] * w;
+				a[i] = u + v;
+				a[i + len / 2] = u - v;
+			}
+		}
+	}
+}
+
+// Device tables of one transform length.
+struct FftDev {
+	int n = 0, m = 0;
+	DevBuf tw, chirp, bhat;
+	Fft1dPlan plan() const
+	{
+		Fft1dPlan p;
+		p.n = n; p.m = m;
+		p.tw = tw.as<float2>();
+		p.chirp = chirp.p ? chirp.as<float2>() : nullptr;
+		p.bhat = bhat.p ? bhat.as<float2>() : nullptr;
+		return p;
+	}
+};
+
+// Round-edge mask (GRoundEdge.cu:14-24) with centre (T/2, T/2), size (T, T).
+double round_edge(int x, int y, int tile)
+{
+	double c = (double)((float)tile * 0.5f), s = (double)((float)tile * 1.0f);
+	double fx = 2.0 * fabs((double)x - c) / s, fy = 2.0 * fabs((double)y - c) / s;
+	double r = sqrt(fx * fx + fy * fy);
+	if (r >= 1.0) return 0.0;
+	double q = 0.5 * (1.0 - cos((double)3.1415926f * r));
+	return 1.0 - q * q * q;
+}
+
+// Tables shared by every plan with tile size T.
+struct TileTables {
+	int T = 0;
+	DevBuf tw, mask, h1, roff, rcnt, ridx, w1d;
+	int nsamples = 0;
+};
+
+struct RescalePlan {
+	int nx = 0, ny = 0;
+	float pix_raw = 0.f, bin = 1.f, pixn = 1.f;
+	int nnx = 0, nny = 0, pitch = 0;
+	bool active = false;
+	FftDev *rows = nullptr, *cf = nullptr, *ci = nullptr, *c2r = nullptr;
+};
+
+struct SearchPlan {
+	float kv = 0, cs = 0, ac = 0, pix = 0;
+	int T = 0;
+	CtfConsts cc{};
+	float lo1d = 0, hi1d = 0, flo2d = 0, fhi2d = 0;
+	float res[2] = {0, 0};
+	RingTables ring{};
+	DevBuf idx, a, b, c2, s2, w;
+};
+
+struct GraphEntry {
+	hipGraphExec_t exec = nullptr;
+	int gen = -1;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct gctf_ctx {
+	int device = 0;
+	hipStream_t own_stream = nullptr;
+	hipStream_t stream = nullptr;
+	gctf_params prm{};
+	std::string err;
+	int chunk = 4, tile_splits = 8, use_graph = 1;
+	int gen = 0;  // bumped whenever a device buffer moves
+
+	std::map<int, std::unique_ptr<FftDev>> ffts;
+	std::map<int, std::unique_ptr<TileTables>> tiles;
+	std::map<std::string, std::unique_ptr<SearchPlan>> splans;
+	std::map<std::string, GraphEntry> graphs;
+
+	// single-image (reference-style) state
+	RescalePlan rp;
+	bool rp_ready = false, rp_done = false;
+	DevBuf pad1;
+	float s_kv = 0, s_cs = 0, s_ac = 0, s_pix = 0;
+	bool setup1 = false, setup2 = false, have_raw = false, have_spect = false;
+	int s_nx = 0, s_ny = 0;
+
+	// batch buffers
+	int cap = 0;  // images the per-image buffers hold
+	DevBuf cbufA, cbufB, padc, bsums, rt, tpart;
+	DevBuf raw, hsum, spect, mom, q, radial, ring_spec, ring_ss, states, part2d, refine, shell_cc, fullsp,
+		moments1;
+	std::vector<SearchState> h_states;
+	SearchState* h_pinned = nullptr;
+	int h_pinned_n = 0;
+
+	~gctf_ctx()
+	{
+		for (auto& kv : graphs)
+			if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+		if (h_pinned) (void)hipHostFree(h_pinned);
+		if (own_stream) (void)hipStreamDestroy(own_stream);
+	}
+};
+
+namespace {
+
+int fail(gctf_ctx* c, int code, const std::string& msg)
+{
+	if (c) c->err = msg;
+	return code;
+}
+
+#define HIPCHK(ctx, call)                                                                    \
+	do {                                                                                     \
+		hipError_t e_ = (call);                                                              \
+		if (e_ != hipSuccess)                                                                \
+			return fail(ctx, e_ == hipErrorOutOfMemory ? GCTF_ERR_NOMEM : GCTF_ERR_HIP,      \
+				std::string(#call) + ": " + hipGetErrorString(e_));                           \
+	} while (0)
+
+#define CHK(expr)                    \
+	do {                             \
+		int r_ = (expr);             \
+		if (r_ != GCTF_OK) return r_; \
+	} while (0)
+
+int ensure_buf(gctf_ctx* c, DevBuf& b, size_t bytes)
+{
+	bool moved = false;
+	hipError_t e = b.ensure(bytes, &moved);
+	if (e != hipSuccess)
+		return fail(c, GCTF_ERR_NOMEM, std::string("device allocation of ") + std::to_string(bytes)
+			+ " bytes failed: " + hipGetErrorString(e));
+	if (moved) c->gen++;
+	return GCTF_OK;
+}
+
+int get_fft(gctf_ctx* c, int n, FftDev** out)
+{
+	auto it = c->ffts.find(n);
+	if (it != c->ffts.end()) { *out = it->second.get(); return GCTF_OK; }
+	auto f = std::make_unique<FftDev>();
+	f->n = n;
+	f->m = is_pow2(n) ? n : next_pow2(2 * n - 1);
+	const int m = f->m;
+	if (m > 16384)
+		return fail(c, GCTF_ERR_UNSUPPORTED, "transform length " + std::to_string(n)
+			+ " needs a working length above 16384");
+	std::vector<float2> tw(m);
+	for (int e = 0; e < m; ++e) {
+		double a = -2.0 * M_PI * (double)e / (double)m;
+		tw[e] = make_float2((float)cos(a), (float)sin(a));
+	}
+	CHK(ensure_buf(c, f->tw, sizeof(float2) * m));
+	HIPCHK(c, hipMemcpy(f->tw.p, tw.data(), sizeof(float2) * m, hipMemcpyHostToDevice));
+	if (!is_pow2(n)) {
+		std::vector<std::complex<double>> w(n), b(m, 0.0);
+		std::vector<float2> chirp(n), bhat(m);
+		for (int k = 0; k < n; ++k) {
+			long long k2 = ((long long)k * k) % (2LL * n);
+			double a = -M_PI * (double)k2 / (double)n;
+			w[k] = std::complex<double>(cos(a), sin(a));
+			chirp[k] = make_float2((float)w[k].real(), (float)w[k].imag());
+		}
+		b[0] = std::conj(w[0]);
+		for (int k = 1; k < n; ++k) { b[k] = std::conj(w[k]); b[m - k] = b[k]; }
+		host_fft(b, false);
+		for (int k = 0; k < m; ++k) bhat[k] = make_float2((float)b[k].real(), (float)b[k].imag());
+		CHK(ensure_buf(c, f->chirp, sizeof(float2) * n));
+		CHK(ensure_buf(c, f->bhat, sizeof(float2) * m));
+		HIPCHK(c, hipMemcpy(f->chirp.p, chirp.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+		HIPCHK(c, hipMemcpy(f->bhat.p, bhat.data(), sizeof(float2) * m, hipMemcpyHostToDevice));
+	}
+	*out = f.get();
+	c->ffts[n] = std::move(f);
+	return GCTF_OK;
+}
+
+// Radial sample table (GRadialAvg.cu:17-31) in IEEE single, the reference's
+// operation order; the row index cmpy reached at r = cmpx-1 is taken mod cmpy.
+void radial_table(int cmpx, int cmpy, std::vector<int>& off, std::vector<int>& cnt,
+	std::vector<int>& idx)
+{
+	off.assign(cmpx, 0);
+	cnt.assign(cmpx, 0);
+	idx.clear();
+	for (int r = 0; r < cmpx; ++r) {
+		float fp = 3.1415926f * (float)r + 0.5f;
+		int np = (int)fp;
+		if (np < 1) np = 1;
+		else if (np > 180) np = 180;
+		float step = 180.0f / (float)np;
+		off[r] = (int)idx.size();
+		cnt[r] = np;
+		for (int i = 0; i < np; ++i) {
+			float rad = (-90.0f + (float)i * step) * 0.017453f;
+			int x = (int)((float)r * cosf(rad) + 0.5f);
+			int y = (int)((float)r * sinf(rad) + (float)(cmpy / 2) + 0.5f);
+			idx.push_back((y % cmpy) * cmpx + x);
+		}
+	}
+}
+
+int get_tile_tables(gctf_ctx* c, int T, TileTables** out)
+{
+	auto it = c->tiles.find(T);
+	if (it != c->tiles.end()) { *out = it->second.get(); return GCTF_OK; }
+	auto t = std::make_unique<TileTables>();
+	t->T = T;
+	const int cx = T / 2 + 1;
+	std::vector<float2> tw(T);
+	for (int e = 0; e < T; ++e) {
+		double a = -2.0 * M_PI * e / T;
+		tw[e] = make_float2((float)cos(a), (float)sin(a));
+	}
+	std::vector<float> mask((size_t)T * T), h1(T);
+	for (int y = 0; y < T; ++y)
+		for (int x = 0; x < T; ++x) mask[(size_t)y * T + x] = (float)round_edge(x, y, T);
+	// GLowpass2D::DoBFactor (GLowpass2D.cu:57-74): fScale = (float)(-10 / (Nx^2 + Ny^2))
+	double dtemp = (double)T * T + (double)T * T;
+	float fscale = (float)(-10.0f / dtemp);
+	for (int k = 0; k < T; ++k) {
+		int kk = k > T / 2 ? k - T : k;
+		h1[k] = (float)exp((double)fscale * (double)(kk * kk));
+	}
+	std::vector<int> off, cnt, idx;
+	radial_table(cx, T, off, cnt, idx);
+	std::vector<double> w1d(cx);
+	for (int x = 0; x < cx; ++x) {
+		double f = (double)x / ((double)cx - 1.0);  // GCC1D.cu:36-39, B = 1
+		w1d[x] = exp(-f * f);
+	}
+	CHK(ensure_buf(c, t->tw, sizeof(float2) * T));
+	CHK(ensure_buf(c, t->mask, sizeof(float) * mask.size()));
+	CHK(ensure_buf(c, t->h1, sizeof(float) * T));
+	CHK(ensure_buf(c, t->roff, sizeof(int) * cx));
+	CHK(ensure_buf(c, t->rcnt, sizeof(int) * cx));
+	CHK(ensure_buf(c, t->ridx, sizeof(int) * idx.size()));
+	CHK(ensure_buf(c, t->w1d, sizeof(double) * cx));
+	HIPCHK(c, hipMemcpy(t->tw.p, tw.data(), sizeof(float2) * T, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(t->mask.p, mask.data(), sizeof(float) * mask.size(), hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(t->h1.p, h1.data(), sizeof(float) * T, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(t->roff.p, off.data(), sizeof(int) * cx, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(t->rcnt.p, cnt.data(), sizeof(int) * cx, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(t->ridx.p, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(t->w1d.p, w1d.data(), sizeof(double) * cx, hipMemcpyHostToDevice));
+	t->nsamples = (int)idx.size();
+	*out = t.get();
+	c->tiles[T] = std::move(t);
+	return GCTF_OK;
+}
+
+// CCTFTheory::Setup (CCTFTheory.cpp:69-86) + the kernels' amplitude phase
+// (GCalcCTF2D.cu:88-97) + resolution range (CFindCtfBase.cpp:64-68) + CC
+// frequency bounds (CFindDefocus1D.cpp:128-135, CFindDefocus2D.cpp:86-92).
+int get_search_plan(gctf_ctx* c, float kv, float cs, float ac, float pix, int T, SearchPlan** out)
+{
+	char key[128];
+	snprintf(key, sizeof key, "%a/%a/%a/%a/%d", kv, cs, ac, pix, T);
+	auto it = c->splans.find(key);
+	if (it != c->splans.end()) { *out = it->second.get(); return GCTF_OK; }
+	auto sp = std::make_unique<SearchPlan>();
+	sp->kv = kv; sp->cs = cs; sp->ac = ac; sp->pix = pix; sp->T = T;
+	float kv1000 = kv * 1000.0f;
+	float wl_a = (float)(12.26 / sqrt((double)kv1000 + 0.9784 * (double)kv * (double)kv));
+	sp->cc.wl = wl_a / pix;
+	sp->cc.cs = (float)((double)cs * 1e7 / (double)pix);
+	sp->cc.amp_phase = atanf(ac / sqrtf(1.0f - ac * ac));
+	sp->cc.pix = pix;
+	sp->res[0] = 25.0f * pix;
+	sp->res[1] = (2.0f * pix) / 0.8f;
+	if (sp->res[1] < 3.5f) sp->res[1] = 3.5f;
+	const int cx = T / 2 + 1, cy = T;
+	float r1 = (float)((cx - 1) * 2) * pix;
+	sp->lo1d = r1 / sp->res[0];
+	sp->hi1d = r1 / sp->res[1];
+	float r2 = (float)cy * pix;
+	sp->flo2d = r2 / sp->res[0];
+	sp->fhi2d = r2 / sp->res[1];
+	// ring (GCC2D.cu:28-33,152-156), float membership test
+	float l = sp->flo2d / (float)cy, h = sp->fhi2d / (float)cy;
+	if (h > 0.75f) h = 0.75f;
+	float lo2 = l * l, hi2 = h * h;
+	std::vector<int> idx;
+	std::vector<float> a, b, c2, s2, w;
+	const double pir = (double)3.1415926f / M_PI;
+	const double wl = sp->cc.wl, csp = sp->cc.cs;
+	for (int y = 0; y < cy; ++y)
+		for (int x = 0; x < cx; ++x) {
+			float fy = ((float)y - (float)cy * 0.5f) / (float)cy;
+			float fx = (0.5f * (float)x) / (float)(cx - 1);
+			float s2f = fx * fx + fy * fy;
+			if (s2f < lo2 || s2f > hi2) continue;
+			double dfx = (double)((float)x * 0.5f / (float)(cx - 1));
+			double dfy = (double)((float)(y - cy / 2) / (float)cy);
+			double ss = dfx * dfx + dfy * dfy;
+			double th = atan(dfy / (dfx + (double)1e-30f));
+			idx.push_back(y * cx + x);
+			a.push_back((float)(pir * wl * ss));
+			b.push_back((float)(pir * 0.5 * wl * wl * wl * ss * ss * csp));
+			c2.push_back((float)cos(2.0 * th));
+			s2.push_back((float)sin(2.0 * th));
+			w.push_back((float)exp(-16.0 * (double)s2f));
+		}
+	const int n = (int)idx.size();
+	if (n == 0) return fail(c, GCTF_ERR_ARG, "empty correlation ring");
+	CHK(ensure_buf(c, sp->idx, sizeof(int) * n));
+	CHK(ensure_buf(c, sp->a, sizeof(float) * n));
+	CHK(ensure_buf(c, sp->b, sizeof(float) * n));
+	CHK(ensure_buf(c, sp->c2, sizeof(float) * n));
+	CHK(ensure_buf(c, sp->s2, sizeof(float) * n));
+	CHK(ensure_buf(c, sp->w, sizeof(float) * n));
+	HIPCHK(c, hipMemcpy(sp->idx.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(sp->a.p, a.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(sp->b.p, b.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(sp->c2.p, c2.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(sp->s2.p, s2.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(sp->w.p, w.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+	sp->ring.n = n;
+	sp->ring.idx = sp->idx.as<int>();
+	sp->ring.a = sp->a.as<float>();
+	sp->ring.b = sp->b.as<float>();
+	sp->ring.c2 = sp->c2.as<float>();
+	sp->ring.s2 = sp->s2.as<float>();
+	sp->ring.w = sp->w.as<float>();
+	*out = sp.get();
+	c->splans[key] = std::move(sp);
+	return GCTF_OK;
+}
+
+// CRescaleImage::Setup sizes (CRescaleImage.cpp:47-73).
+int make_rescale_plan(gctf_ctx* c, int nx, int ny, float pix, RescalePlan& rp)
+{
+	if (nx < 2 || ny < 2 || !(pix > 0)) return fail(c, GCTF_ERR_ARG, "bad image size or pixel size");
+	rp.nx = nx; rp.ny = ny; rp.pix_raw = pix;
+	float bin = 1.2f / pix;
+	if (bin <= 1) bin = 1.0f;
+	rp.bin = bin;
+	rp.pixn = pix * bin;
+	int nnx = (int)((float)nx / bin + 0.5f), nny = (int)((float)ny / bin + 0.5f);
+	rp.nnx = nnx / 2 * 2;
+	rp.nny = nny / 2 * 2;
+	rp.pitch = (rp.nnx / 2 + 1) * 2;
+	rp.active = bin > 1;
+	if (rp.active) {
+		CHK(get_fft(c, nx, &rp.rows));
+		CHK(get_fft(c, ny, &rp.cf));
+		CHK(get_fft(c, rp.nny, &rp.ci));
+		CHK(get_fft(c, rp.nnx, &rp.c2r));
+	}
+	return GCTF_OK;
+}
+
+TileGrid make_grid(int nx, int ny, int T)
+{
+	TileGrid g{};
+	int ov = (int)((float)T * 0.50f);
+	ov = ov / 2 * 2;
+	int step = T - ov;
+	g.tile = T;
+	g.step = step;
+	g.ntx = (nx - ov) / step;
+	g.nty = (ny - ov) / step;
+	g.offx = (nx - g.ntx * T + (g.ntx - 1) * ov) / 2;
+	g.offy = (ny - g.nty * T + (g.nty - 1) * ov) / 2;
+	g.blocks = (2 * step == T) ? 1 : 0;
+	if (g.blocks) { g.nbx = g.ntx + 1; g.nby = g.nty + 1; g.bsize = step; }
+	else { g.nbx = g.ntx; g.nby = g.nty; g.bsize = T; }
+	return g;
+}
+
+int check_tile(gctf_ctx* c, int T)
+{
+	if (T != 256 && T != 512 && T != 1024)
+		return fail(c, GCTF_ERR_UNSUPPORTED, "tile size " + std::to_string(T)
+			+ " not supported (256, 512 or 1024)");
+	return GCTF_OK;
+}
+
+// Per-image buffers for `n` images of tile size T.
+int ensure_image_buffers(gctf_ctx* c, int n, int T, int ring_n)
+{
+	const size_t cx = T / 2 + 1, npix = cx * T;
+	const int nparts = (int)((npix + 255) / 256);
+	CHK(ensure_buf(c, c->raw, sizeof(float) * npix * n));
+	CHK(ensure_buf(c, c->hsum, sizeof(float) * npix * n));
+	CHK(ensure_buf(c, c->spect, sizeof(float) * npix * n));
+	CHK(ensure_buf(c, c->q, sizeof(float) * npix * n));
+	CHK(ensure_buf(c, c->mom, sizeof(double) * 2 * nparts * n));
+	CHK(ensure_buf(c, c->radial, sizeof(float) * cx * n));
+	CHK(ensure_buf(c, c->ring_spec, sizeof(float) * (size_t)ring_n * n));
+	CHK(ensure_buf(c, c->ring_ss, sizeof(double) * n));
+	CHK(ensure_buf(c, c->part2d, sizeof(double2) * (size_t)kMaxCands * kPixChunks * n));
+	CHK(ensure_buf(c, c->refine, sizeof(RefineArgs) * n));
+	CHK(ensure_buf(c, c->shell_cc, sizeof(float) * cx * n));
+	size_t old = c->states.bytes;
+	CHK(ensure_buf(c, c->states, sizeof(SearchState) * n));
+	if (c->states.bytes != old) {
+		// new state storage starts zeroed (CFindCtfBase / CFindDefocus2D constructors)
+		std::vector<SearchState> z(c->states.bytes / sizeof(SearchState));
+		memset(z.data(), 0, sizeof(SearchState) * z.size());
+		HIPCHK(c, hipMemcpy(c->states.p, z.data(), sizeof(SearchState) * z.size(), hipMemcpyHostToDevice));
+	}
+	if (c->h_pinned_n < n) {
+		if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+		c->h_pinned = nullptr;
+		HIPCHK(c, hipHostMalloc((void**)&c->h_pinned, sizeof(SearchState) * n, hipHostMallocDefault));
+		c->h_pinned_n = n;
+	}
+	return GCTF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launch sequences
+// ---------------------------------------------------------------------------
+
+// Rescale n images (CRescaleImage::DoIt) into padded rows at out + i * out_stride.
+template <typename Tin>
+int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long long in_stride, int n,
+	float* out, long long out_stride)
+{
+	hipStream_t s = c->stream;
+	const int ncx = rp.nnx / 2 + 1;
+	const long long sa = (long long)rp.ny * ncx;       // rows x cropped columns
+	const long long sb = (long long)ncx * rp.nny;
+	float2* A = c->cbufA.as<float2>();
+	float2* B = c->cbufB.as<float2>();
+	const long long stA = std::max(sa, sb), stB = std::max(sa, sb);
+	// 1. rows R2C with x crop
+	{
+		const int m = rp.rows->m;
+		dim3 grid((rp.ny + 1) / 2, n);
+		Fft1dPlan p = rp.rows->plan();
+		if (m <= 2048) k_rescale_rows_r2c<Tin, 256, 2048><<<grid, 256, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
+		else if (m <= 4096) k_rescale_rows_r2c<Tin, 512, 4096><<<grid, 512, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
+		else if (m <= 8192) k_rescale_rows_r2c<Tin, 1024, 8192><<<grid, 1024, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
+		else k_rescale_rows_r2c<Tin, 1024, 16384><<<grid, 1024, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
+	}
+	// 2. transpose (ny x ncx) -> (ncx x ny)
+	{
+		dim3 grid((ncx + 31) / 32, (rp.ny + 31) / 32, n);
+		k_transpose_c<<<grid, 256, 0, s>>>(A, stA, rp.ny, ncx, B, stB);
+	}
+	// 3. columns: forward ny, crop, inverse nny
+	{
+		const int m = std::max(rp.cf->m, rp.ci->m);
+		dim3 grid(ncx, n);
+		Fft1dPlan pf = rp.cf->plan(), pi = rp.ci->plan();
+		if (m <= 2048) k_rescale_cols<256, 2048><<<grid, 256, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
+		else if (m <= 4096) k_rescale_cols<512, 4096><<<grid, 512, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
+		else if (m <= 8192) k_rescale_cols<1024, 8192><<<grid, 1024, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
+		else k_rescale_cols<1024, 16384><<<grid, 1024, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
+	}
+	// 4. transpose (ncx x nny) -> (nny x ncx)
+	{
+		dim3 grid((rp.nny + 31) / 32, (ncx + 31) / 32, n);
+		k_transpose_c<<<grid, 256, 0, s>>>(A, stA, ncx, rp.nny, B, stB);
+	}
+	// 5. rows C2R into the padded image
+	{
+		const int m = rp.c2r->m;
+		dim3 grid((rp.nny + 1) / 2, n);
+		Fft1dPlan p = rp.c2r->plan();
+		if (m <= 2048) k_rescale_rows_c2r<256, 2048><<<grid, 256, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
+		else if (m <= 4096) k_rescale_rows_c2r<512, 4096><<<grid, 512, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
+		else if (m <= 8192) k_rescale_rows_c2r<1024, 8192><<<grid, 1024, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
+		else k_rescale_rows_c2r<1024, 16384><<<grid, 1024, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
+	}
+	HIPCHK(c, hipGetLastError());
+	return GCTF_OK;
+}
+
+int ensure_rescale_buffers(gctf_ctx* c, const RescalePlan& rp, int n)
+{
+	const size_t ncx = rp.nnx / 2 + 1;
+	const size_t per = std::max((size_t)rp.ny * ncx, ncx * (size_t)rp.nny);
+	CHK(ensure_buf(c, c->cbufA, sizeof(float2) * per * n));
+	CHK(ensure_buf(c, c->cbufB, sizeof(float2) * per * n));
+	return GCTF_OK;
+}
+
+int launch_rescale(gctf_ctx* c, const RescalePlan& rp, const void* in, int dtype, long long in_stride,
+	int n, float* out, long long out_stride)
+{
+	if (dtype == GCTF_DTYPE_F16)
+		return launch_rescale_t<_Float16>(c, rp, (const _Float16*)in, in_stride, n, out, out_stride);
+	return launch_rescale_t<float>(c, rp, (const float*)in, in_stride, n, out, out_stride);
+}
+
+int ensure_spectrum_buffers(gctf_ctx* c, const TileGrid& g, int T, int n)
+{
+	const size_t cx = T / 2 + 1;
+	const size_t ntiles = (size_t)g.ntx * g.nty;
+	CHK(ensure_buf(c, c->bsums, sizeof(double) * (size_t)g.nbx * g.nby * n));
+	CHK(ensure_buf(c, c->rt, sizeof(float2) * ntiles * cx * T * n));
+	CHK(ensure_buf(c, c->tpart, sizeof(float) * (size_t)c->tile_splits * T * cx * n));
+	return GCTF_OK;
+}
+
+// Tile-averaged amplitude spectra (CGenAvgSpectrum::DoIt) of n images into
+// raw[slot0 .. slot0+n).
+template <typename Tin, int T>
+int launch_spectrum_t(gctf_ctx* c, const TileGrid& g, const TileTables& tt, const Tin* img, int pitch,
+	long long stride, int n, int slot0)
+{
+	hipStream_t s = c->stream;
+	constexpr int CX = T / 2 + 1;
+	const int ntiles = g.ntx * g.nty;
+	const int nsplit = std::min(c->tile_splits, ntiles);
+	k_block_sums<Tin><<<dim3(g.nbx * g.nby, n), 256, 0, s>>>(img, stride, pitch, g, c->bsums.as<double>());
+	constexpr int ROWS = 16;
+	const long long rts = (long long)ntiles * CX * T;
+	k_tile_rows<Tin, T, ROWS><<<dim3(ntiles * (T / ROWS), n), (ROWS / 2) * (T / 8), 0, s>>>(
+		img, stride, pitch, g, c->bsums.as<double>(), tt.mask.as<float>(), tt.tw.as<float2>(),
+		c->rt.as<float2>(), rts);
+	constexpr int CPW = 4;
+	const float fac = 1.0f / (float)ntiles;                   // CGenAvgSpectrum.cpp:90
+	k_tile_cols<T, CPW><<<dim3((CX + CPW - 1) / CPW, nsplit, n), CPW * (T / 8), 0, s>>>(
+		c->rt.as<float2>(), rts, ntiles, nsplit, fac, tt.tw.as<float2>(), c->tpart.as<float>());
+	const int npix = T * CX;
+	k_tile_reduce<<<dim3((npix + 255) / 256, n), 256, 0, s>>>(c->tpart.as<float>(), nsplit, npix,
+		c->raw.as<float>() + (size_t)slot0 * npix);
+	HIPCHK(c, hipGetLastError());
+	return GCTF_OK;
+}
+
+template <typename Tin>
+int launch_spectrum(gctf_ctx* c, const TileGrid& g, const TileTables& tt, const Tin* img, int pitch,
+	long long stride, int n, int slot0)
+{
+	switch (g.tile) {
+	case 256: return launch_spectrum_t<Tin, 256>(c, g, tt, img, pitch, stride, n, slot0);
+	case 512: return launch_spectrum_t<Tin, 512>(c, g, tt, img, pitch, stride, n, slot0);
+	case 1024: return launch_spectrum_t<Tin, 1024>(c, g, tt, img, pitch, stride, n, slot0);
+	default: return fail(c, GCTF_ERR_UNSUPPORTED, "tile size");
+	}
+}
+
+// Background removal, flatness gate and low-pass (CFindCtfBase::
+// mRemoveBackground / mLowpass) for images [0, n).
+template <int T>
+int launch_post_t(gctf_ctx* c, const TileTables& tt, int n)
+{
+	hipStream_t s = c->stream;
+	constexpr int CX = T / 2 + 1;
+	const int npix = T * CX;
+	// box side (GRmBackground2D.cu:33-36) with fMinFreq = 1/15
+	int box = (int)((1.0f / 15.0f) * (float)T);
+	if ((float)box > 0.3f * (float)T) box = (int)(0.3f * (float)T);
+	box = box / 2 * 2 + 1;
+	if (box < 7) box = 7;
+	const int hb = box / 2;
+	const int ls = c->prm.log_spect ? 1 : 0;
+	k_bg_rows<T><<<dim3(T, n), 256, 0, s>>>(c->raw.as<float>(), ls, hb, c->hsum.as<float>());
+	const int nparts = (npix + 255) / 256;
+	k_bg_cols<T><<<dim3(nparts, 1, n), 256, 0, s>>>(c->raw.as<float>(), c->hsum.as<float>(), ls, hb,
+		c->spect.as<float>(), c->mom.as<double>());
+	k_gate<<<n, 256, 0, s>>>(c->mom.as<double>(), nparts, npix, c->states.as<SearchState>());
+	k_lp_x<T><<<dim3(T / 4, n), 2 * (T / 8), 0, s>>>(c->spect.as<float>(), c->states.as<SearchState>(),
+		tt.h1.as<float>(), tt.tw.as<float2>(), c->q.as<float>());
+	k_lp_y<T><<<dim3((CX + 1) / 2, n), T / 8, 0, s>>>(c->q.as<float>(), c->states.as<SearchState>(),
+		tt.h1.as<float>(), tt.tw.as<float2>(), c->spect.as<float>());
+	HIPCHK(c, hipGetLastError());
+	return GCTF_OK;
+}
+
+int launch_post(gctf_ctx* c, const TileTables& tt, int n)
+{
+	switch (tt.T) {
+	case 256: return launch_post_t<256>(c, tt, n);
+	case 512: return launch_post_t<512>(c, tt, n);
+	case 1024: return launch_post_t<1024>(c, tt, n);
+	default: return fail(c, GCTF_ERR_UNSUPPORTED, "tile size");
+	}
+}
+
+// Radial average + CFindCtf1D::Do1D / Refine1D for images [0, n).
+int launch_1d(gctf_ctx* c, const SearchPlan& sp, const TileTables& tt, int n, int mode, float init_df,
+	float range)
+{
+	hipStream_t s = c->stream;
+	const int T = sp.T, cx = T / 2 + 1;
+	k_radial<<<n, 256, 0, s>>>(c->spect.as<float>(), cx, T, tt.roff.as<int>(), tt.rcnt.as<int>(),
+		tt.ridx.as<int>(), c->radial.as<float>());
+	Search1DParams p;
+	p.c = sp.cc; p.cmpx = cx; p.lo = sp.lo1d; p.hi = sp.hi1d;
+	p.mode = mode; p.init_df = init_df; p.range = range;
+	k_search1d<<<n, 256, 0, s>>>(p, c->radial.as<float>(), tt.w1d.as<double>(), c->states.as<SearchState>());
+	HIPCHK(c, hipGetLastError());
+	return GCTF_OK;
+}
+
+// The CFindDefocus2D::mIterate stage program + mCalcCtfRes for images [0, n).
+int launch_2d(gctf_ctx* c, const SearchPlan& sp, int n, int begin, float phase_range)
+{
+	hipStream_t s = c->stream;
+	const int T = sp.T, cx = T / 2 + 1;
+	SearchState* st = c->states.as<SearchState>();
+	double2* part = c->part2d.as<double2>();
+	const double* ss = c->ring_ss.as<double>();
+	k_ring_gather<<<n, 1024, 0, s>>>(c->spect.as<float>(), T * cx, sp.ring, c->ring_spec.as<float>(),
+		c->ring_ss.as<double>());
+	constexpr int CPB = 8;
+	auto eval = [&](int maxc) {
+		dim3 grid((maxc + CPB - 1) / CPB, kPixChunks, n);
+		k_cc2d_eval<CPB><<<grid, 256, 0, s>>>(sp.cc, sp.ring, c->ring_spec.as<float>(), st, part);
+	};
+	auto step = [&](int next) {
+		k_s2d_step<<<n, 256, 0, s>>>(st, part, ss, next, c->refine.as<RefineArgs>());
+	};
+	int maxph = 61;
+	if (phase_range > 61.0f) maxph = std::min(kMaxCands, (int)phase_range + 3);
+	step(begin);
+	eval(1);
+	step(NX_ASTIG);
+	eval(kMaxCands);
+	step(NX_ASTMAG);
+	for (int i = 1; i < 20; ++i) {
+		eval(21);
+		step(NX_ASTANG);
+		eval(51);
+		step(NX_DFMEAN);
+		eval(31);
+		step(NX_PHASE);
+		if (phase_range > 0.0f) eval(maxph);
+		step(i < 19 ? NX_ASTMAG : NX_FINISH);
+	}
+	k_spectral_cc<<<dim3(cx, n), 256, 0, s>>>(sp.cc, cx, T, 10, c->spect.as<float>(), st,
+		c->shell_cc.as<float>());
+	k_ctf_res<<<(n + 63) / 64, 64, 0, s>>>(c->shell_cc.as<float>(), cx, T, sp.pix, st, n);
+	HIPCHK(c, hipGetLastError());
+	return GCTF_OK;
+}
+
+__global__ void k_state_init(SearchState* st, int n, float p0, float range)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	SearchState s;
+	memset(&s, 0, sizeof s);
+	s.ext_phase = p0;          // CFindCtfBase::SetPhase
+	s.phase_range = range;
+	st[i] = s;
+}
+
+int read_results(gctf_ctx* c, int n, gctf_result* out)
+{
+	HIPCHK(c, hipMemcpyAsync(c->h_pinned, c->states.p, sizeof(SearchState) * n, hipMemcpyDeviceToHost,
+		c->stream));
+	HIPCHK(c, hipStreamSynchronize(c->stream));
+	for (int i = 0; i < n; ++i) {
+		const SearchState& s = c->h_pinned[i];
+		out[i].df_min = s.df_min;
+		out[i].df_max = s.df_max;
+		out[i].azimuth_deg = s.ast_ang;
+		out[i].ext_phase_deg = s.ext_phase;
+		out[i].score = s.score;
+		out[i].ctf_res = s.ctf_res;
+	}
+	return GCTF_OK;
+}
+
+// ---- diagnostic full spectrum (CFindCtfBase::GenFullSpectrum) -------------
+__global__ __launch_bounds__(256) void k_moments1(const float* __restrict__ sp, int n, double* out)
+{
+	__shared__ double r1[256], r2[256];
+	double a = 0.0, b = 0.0;
+	for (int i = threadIdx.x; i < n; i += 256) { double v = sp[i]; a += v; b += v * v; }
+	r1[threadIdx.x] = a; r2[threadIdx.x] = b;
+	__syncthreads();
+	for (int o = 128; o > 0; o >>= 1) {
+		if (threadIdx.x < o) { r1[threadIdx.x] += r1[threadIdx.x + o]; r2[threadIdx.x] += r2[threadIdx.x + o]; }
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) { out[0] = r1[0] / n; out[1] = r2[0] / n; }
+}
+
+// CSpectrumImage::DoIt (CSpectrumImage.cpp:21-71): unpadded unfold
+// (GCalcSpectrum.cu:47-69, row cmpy taken mod cmpy) and |CTF| painted on the
+// left half (GCalcCTF2D.cu:49-78).  Grid (T), 256 threads.
+__global__ __launch_bounds__(256) void k_full_spectrum(const float* __restrict__ half, int cmpx, int cmpy,
+	CtfConsts cc, float dfmin_pix, float dfmax_pix, float az, float add, float fmin, float fmax,
+	const double* __restrict__ mom, float* __restrict__ out)
+{
+	const int hx = cmpx - 1, nxf = 2 * hx, y = blockIdx.x;
+	const double mean = mom[0], var = mom[1] - mom[0] * mom[0];
+	const float sd = var < 0 ? 0.0f : (float)sqrt(var);
+	const float gain = sd * 1.5f;
+	const float dmean = 0.5f * (dfmin_pix + dfmax_pix), dsig = 0.5f * (dfmax_pix - dfmin_pix);
+	const float w2 = cc.wl * cc.wl;
+	for (int xd = threadIdx.x; xd < nxf; xd += 256) {
+		int xs = xd - hx, ys = y;
+		if (xs < 0) { xs = -xs; ys = (cmpy - y) % cmpy; }
+		float v = half[ys * cmpx + xs];
+		if (xd < hx) {
+			float fy = ((float)y - (float)cmpy * 0.5f) / (float)cmpy;
+			float fx = ((float)xd - (float)hx) * 0.5f / (float)hx;
+			fx = sqrtf(fx * fx + fy * fy);
+			if (!(fx < fmin || fx > fmax)) {
+				int ix = hx - xd, iy = (cmpy - y) % cmpy;
+				float fX = (float)ix * 0.5f / (float)(cmpx - 1);
+				float fY = (float)(iy - cmpy / 2) / (float)cmpy;
+				float s2 = fX * fX + fY * fY;
+				float th = atanf(fY / (fX + (float)1e-30));
+				float df = dmean + dsig * cosf(2.0f * (th - az));
+				float ctf = -sinf(add + kPi2D * cc.wl * s2 * (df - 0.5f * s2 * w2 * cc.cs));
+				v = (float)(((double)fabsf(ctf) - 0.5) * gain + mean);
+			}
+		}
+		out[(long long)y * nxf + xd] = v;
+	}
+}
+
+int need(gctf_ctx* c, bool cond, const char* what)
+{
+	if (!cond) return fail(c, GCTF_ERR_STATE, what);
+	return GCTF_OK;
+}
+
+int current_plan(gctf_ctx* c, SearchPlan** sp, TileTables** tt)
+{
+	CHK(need(c, c->setup1, "Setup1 has not been called"));
+	CHK(get_search_plan(c, c->s_kv, c->s_cs, c->s_ac, c->s_pix, c->prm.tile_size, sp));
+	CHK(get_tile_tables(c, c->prm.tile_size, tt));
+	CHK(ensure_image_buffers(c, 1, c->prm.tile_size, (*sp)->ring.n));
+	return GCTF_OK;
+}
+
+int single_result(gctf_ctx* c, gctf_result* out)
+{
+	gctf_result tmp;
+	CHK(read_results(c, 1, &tmp));
+	if (out) *out = tmp;
+	return GCTF_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int gctf_create(const gctf_params* params, int device, gctf_ctx** out)
+{
+	if (!out) return GCTF_ERR_ARG;
+	*out = nullptr;
+	gctf_params p;
+	if (params) p = *params;
+	else { p.kv = 300.f; p.cs_mm = 2.7f; p.amp_contrast = 0.07f; p.ext_phase_deg = 0.f;
+		p.ext_phase_range_deg = 0.f; p.tile_size = 512; p.log_spect = 0; }
+	if (!(p.kv > 0) || !(p.amp_contrast >= 0 && p.amp_contrast < 1)) return GCTF_ERR_ARG;
+	if (p.tile_size != 256 && p.tile_size != 512 && p.tile_size != 1024) return GCTF_ERR_UNSUPPORTED;
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return GCTF_ERR_HIP;
+	if (hipSetDevice(device) != hipSuccess) return GCTF_ERR_HIP;
+	auto c = std::make_unique<gctf_ctx>();
+	c->device = device;
+	c->prm = p;
+	if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) return GCTF_ERR_HIP;
+	c->stream = c->own_stream;
+	*out = c.release();
+	return GCTF_OK;
+}
+
+void gctf_destroy(gctf_ctx* ctx)
+{
+	if (!ctx) return;
+	(void)hipSetDevice(ctx->device);
+	(void)hipStreamSynchronize(ctx->stream);
+	delete ctx;
+}
+
+const char* gctf_error_string(const gctf_ctx* ctx)
+{
+	if (!ctx) return "null context";
+	return ctx->err.c_str();
+}
+
+int gctf_set_stream(gctf_ctx* ctx, void* hip_stream)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+	for (auto& kv : ctx->graphs)
+		if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+	ctx->graphs.clear();
+	return GCTF_OK;
+}
+
+int gctf_synchronize(gctf_ctx* ctx)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	return GCTF_OK;
+}
+
+int gctf_tune(gctf_ctx* ctx, int chunk, int tile_splits, int use_graph)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	if (chunk > 0) ctx->chunk = chunk;
+	if (tile_splits > 0) { ctx->tile_splits = tile_splits; ctx->gen++; }
+	if (use_graph != 0) ctx->use_graph = use_graph > 0 ? 1 : 0;
+	return GCTF_OK;
+}
+
+int gctf_rescale_setup(gctf_ctx* ctx, int nx, int ny, float raw_pix, int* new_size, float* new_pix)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	ctx->rp_ready = false;
+	ctx->rp_done = false;
+	CHK(make_rescale_plan(ctx, nx, ny, raw_pix, ctx->rp));
+	CHK(ensure_buf(ctx, ctx->pad1, sizeof(float) * (size_t)ctx->rp.pitch * ctx->rp.nny));
+	if (ctx->rp.active) CHK(ensure_rescale_buffers(ctx, ctx->rp, 1));
+	ctx->rp_ready = true;
+	if (new_size) { new_size[0] = ctx->rp.nnx; new_size[1] = ctx->rp.nny; new_size[2] = ctx->rp.pitch; }
+	if (new_pix) *new_pix = ctx->rp.pixn;
+	return GCTF_OK;
+}
+
+int gctf_rescale(gctf_ctx* ctx, const float* img, int img_on_device)
+{
+	if (!ctx || !img) return fail(ctx, GCTF_ERR_ARG, "null image");
+	CHK(need(ctx, ctx->rp_ready, "CRescaleImage::Setup has not been called"));
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	const RescalePlan& rp = ctx->rp;
+	float* out = ctx->pad1.as<float>();
+	if (!rp.active) {
+		// CRescaleImage.cpp:76-85: rows copied into the padded buffer
+		HIPCHK(ctx, hipMemcpy2DAsync(out, sizeof(float) * rp.pitch, img, sizeof(float) * rp.nx,
+			sizeof(float) * rp.nx, rp.nny, img_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+			ctx->stream));
+	} else {
+		const float* dimg = img;
+		DevBuf tmp;
+		if (!img_on_device) {
+			CHK(ensure_buf(ctx, tmp, sizeof(float) * (size_t)rp.nx * rp.ny));
+			HIPCHK(ctx, hipMemcpyAsync(tmp.p, img, sizeof(float) * (size_t)rp.nx * rp.ny,
+				hipMemcpyHostToDevice, ctx->stream));
+			dimg = tmp.as<float>();
+		}
+		CHK(launch_rescale(ctx, rp, dimg, GCTF_DTYPE_F32, (long long)rp.nx * rp.ny, 1, out,
+			(long long)rp.pitch * rp.nny));
+		HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	}
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	ctx->rp_done = true;
+	return GCTF_OK;
+}
+
+int gctf_scaled_image(gctf_ctx* ctx, float** dev_pad_img)
+{
+	if (!ctx || !dev_pad_img) return GCTF_ERR_ARG;
+	CHK(need(ctx, ctx->rp_ready, "CRescaleImage::Setup has not been called"));
+	*dev_pad_img = ctx->pad1.as<float>();
+	return GCTF_OK;
+}
+
+int gctf_setup1(gctf_ctx* ctx, float kv, float cs_mm, float amp_contrast, float pix, float ast_tol,
+	float ext_phase_rad)
+{
+	(void)ast_tol;        // CCTFTheory's AstTol is not used by the search
+	(void)ext_phase_rad;  // kept in CCTFTheory; the search phase comes from SetPhase
+	if (!ctx) return GCTF_ERR_ARG;
+	if (!(kv > 0) || !(pix > 0) || !(amp_contrast >= 0 && amp_contrast < 1))
+		return fail(ctx, GCTF_ERR_ARG, "bad CTF parameters");
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	ctx->s_kv = kv; ctx->s_cs = cs_mm; ctx->s_ac = amp_contrast; ctx->s_pix = pix;
+	ctx->setup1 = true;
+	ctx->have_raw = ctx->have_spect = false;
+	SearchPlan* sp;
+	TileTables* tt;
+	CHK(current_plan(ctx, &sp, &tt));
+	// a fresh CFindDefocus2D (CFindCtf2D::Setup1): astigmatism, angle and its
+	// extra phase start at 0; CFindCtfBase's fields persist.
+	HIPCHK(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->states.p, sizeof(SearchState), hipMemcpyDeviceToHost, ctx->stream));
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	SearchState s = ctx->h_pinned[0];
+	s.ast = 0.f; s.ang = 0.f; s.phase = 0.f; s.done = 0; s.st_kind = ST_NONE; s.st_n = 0;
+	ctx->h_pinned[0] = s;
+	HIPCHK(ctx, hipMemcpyAsync(ctx->states.p, ctx->h_pinned, sizeof(SearchState), hipMemcpyHostToDevice, ctx->stream));
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	return GCTF_OK;
+}
+
+int gctf_setup2(gctf_ctx* ctx, int nx, int ny)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	CHK(need(ctx, ctx->setup1, "Setup1 has not been called"));
+	const int T = ctx->prm.tile_size;
+	if (nx < T || ny < T) return fail(ctx, GCTF_ERR_ARG, "image smaller than one tile");
+	ctx->s_nx = nx; ctx->s_ny = ny;
+	ctx->setup2 = true;
+	return GCTF_OK;
+}
+
+int gctf_set_phase(gctf_ctx* ctx, float p0_deg, float range_deg)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	CHK(need(ctx, ctx->setup1, "Setup1 has not been called"));
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	HIPCHK(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->states.p, sizeof(SearchState), hipMemcpyDeviceToHost, ctx->stream));
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	ctx->h_pinned[0].ext_phase = p0_deg;
+	ctx->h_pinned[0].phase_range = range_deg;
+	HIPCHK(ctx, hipMemcpyAsync(ctx->states.p, ctx->h_pinned, sizeof(SearchState), hipMemcpyHostToDevice, ctx->stream));
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	return GCTF_OK;
+}
+
+int gctf_gen_half_spectrum(gctf_ctx* ctx, const float* dev_pad_img)
+{
+	if (!ctx || !dev_pad_img) return fail(ctx, GCTF_ERR_ARG, "null image");
+	CHK(need(ctx, ctx->setup2, "Setup2 has not been called"));
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	SearchPlan* sp;
+	TileTables* tt;
+	CHK(current_plan(ctx, &sp, &tt));
+	const int T = ctx->prm.tile_size;
+	TileGrid g = make_grid(ctx->s_nx, ctx->s_ny, T);
+	if (g.ntx < 1 || g.nty < 1) return fail(ctx, GCTF_ERR_ARG, "no tile fits the image");
+	CHK(ensure_spectrum_buffers(ctx, g, T, 1));
+	const int pitch = (ctx->s_nx / 2 + 1) * 2;   // CGenAvgSpectrum.cpp:130
+	CHK(launch_spectrum<float>(ctx, g, *tt, dev_pad_img, pitch, 0, 1, 0));
+	CHK(launch_post(ctx, *tt, 1));
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	ctx->have_raw = ctx->have_spect = true;
+	return GCTF_OK;
+}
+
+int gctf_do1d(gctf_ctx* ctx, gctf_result* out)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	CHK(need(ctx, ctx->have_spect, "no spectrum: call GenHalfSpectrum or SetHalfSpect"));
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	SearchPlan* sp;
+	TileTables* tt;
+	CHK(current_plan(ctx, &sp, &tt));
+	CHK(launch_1d(ctx, *sp, *tt, 1, 0, 0.f, 0.f));
+	return single_result(ctx, out);
+}
+
+int gctf_refine1d(gctf_ctx* ctx, float init_df, float df_range, gctf_result* out)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	CHK(need(ctx, ctx->have_spect, "no spectrum: call GenHalfSpectrum or SetHalfSpect"));
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	SearchPlan* sp;
+	TileTables* tt;
+	CHK(current_plan(ctx, &sp, &tt));
+	CHK(launch_1d(ctx, *sp, *tt, 1, 1, init_df, df_range));
+	return single_result(ctx, out);
+}
+
+int gctf_do2d(gctf_ctx* ctx, gctf_result* out)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	CHK(need(ctx, ctx->have_spect, "no spectrum: call GenHalfSpectrum or SetHalfSpect"));
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	SearchPlan* sp;
+	TileTables* tt;
+	CHK(current_plan(ctx, &sp, &tt));
+	// the phase range in use is the one SetPhase stored in the state
+	HIPCHK(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->states.p, sizeof(SearchState), hipMemcpyDeviceToHost, ctx->stream));
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	const float prange = ctx->h_pinned[0].phase_range;
+	CHK(launch_1d(ctx, *sp, *tt, 1, 0, 0.f, 0.f));
+	CHK(launch_2d(ctx, *sp, 1, NX_BEGIN_DO2D, prange));
+	return single_result(ctx, out);
+}
+
+int gctf_refine(gctf_ctx* ctx, const float* df_mean, const float* ast_ratio, const float* ast_angle_deg,
+	const float* ext_phase_deg, gctf_result* out)
+{
+	if (!ctx || !df_mean || !ast_ratio || !ast_angle_deg || !ext_phase_deg)
+		return fail(ctx, GCTF_ERR_ARG, "null refine range");
+	CHK(need(ctx, ctx->have_spect, "no spectrum: call GenHalfSpectrum or SetHalfSpect"));
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	SearchPlan* sp;
+	TileTables* tt;
+	CHK(current_plan(ctx, &sp, &tt));
+	RefineArgs r;
+	for (int i = 0; i < 2; ++i) {
+		r.dfmean[i] = df_mean[i]; r.ast[i] = ast_ratio[i];
+		r.ang[i] = ast_angle_deg[i]; r.phase[i] = ext_phase_deg[i];
+	}
+	HIPCHK(ctx, hipMemcpyAsync(ctx->refine.p, &r, sizeof r, hipMemcpyHostToDevice, ctx->stream));
+	CHK(launch_2d(ctx, *sp, 1, NX_BEGIN_REFINE, ext_phase_deg[1]));
+	return single_result(ctx, out);
+}
+
+int gctf_spect_size(gctf_ctx* ctx, int* size, int half)
+{
+	if (!ctx || !size) return GCTF_ERR_ARG;
+	const int T = ctx->prm.tile_size;
+	size[0] = half ? T / 2 + 1 : T;   // CFindCtfBase.cpp:101-107
+	size[1] = T;
+	return GCTF_OK;
+}
+
+int gctf_get_half_spect(gctf_ctx* ctx, int raw, float* host_out)
+{
+	if (!ctx || !host_out) return fail(ctx, GCTF_ERR_ARG, "null output");
+	CHK(need(ctx, raw ? ctx->have_raw : ctx->have_spect, "no spectrum"));
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	const int T = ctx->prm.tile_size;
+	const size_t n = (size_t)(T / 2 + 1) * T;
+	HIPCHK(ctx, hipMemcpyAsync(host_out, raw ? ctx->raw.p : ctx->spect.p, sizeof(float) * n,
+		hipMemcpyDeviceToHost, ctx->stream));
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	return GCTF_OK;
+}
+
+int gctf_set_half_spect(gctf_ctx* ctx, const float* host_spect)
+{
+	if (!ctx || !host_spect) return fail(ctx, GCTF_ERR_ARG, "null spectrum");
+	CHK(need(ctx, ctx->setup1, "Setup1 has not been called"));
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	SearchPlan* sp;
+	TileTables* tt;
+	CHK(current_plan(ctx, &sp, &tt));
+	const int T = ctx->prm.tile_size;
+	const size_t n = (size_t)(T / 2 + 1) * T;
+	HIPCHK(ctx, hipMemcpyAsync(ctx->spect.p, host_spect, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	ctx->have_spect = true;
+	return GCTF_OK;
+}
+
+int gctf_full_spectrum(gctf_ctx* ctx, float* host_out)
+{
+	if (!ctx || !host_out) return fail(ctx, GCTF_ERR_ARG, "null output");
+	CHK(need(ctx, ctx->have_spect, "no spectrum"));
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	SearchPlan* sp;
+	TileTables* tt;
+	CHK(current_plan(ctx, &sp, &tt));
+	const int T = ctx->prm.tile_size, cx = T / 2 + 1;
+	CHK(ensure_buf(ctx, ctx->fullsp, sizeof(float) * (size_t)T * T));
+	CHK(ensure_buf(ctx, ctx->moments1, sizeof(double) * 2));
+	HIPCHK(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->states.p, sizeof(SearchState), hipMemcpyDeviceToHost, ctx->stream));
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	const SearchState s = ctx->h_pinned[0];
+	// CFindCtfBase::GenFullSpectrum (CFindCtfBase.cpp:118-133) through
+	// CCTFTheory::SetDefocus / SetExtPhase (CCTFTheory.cpp:93-118)
+	const float pi_f = (float)(4.0 * atan(1.0));
+	const float az = s.ast_ang * 0.017453f;
+	const float dmin = s.df_min / sp->pix, dmax = s.df_max / sp->pix;
+	const float ext = fmodf(s.ext_phase * 0.01745329f, pi_f);
+	const float add = sp->cc.amp_phase + ext;
+	const float fmin = sp->pix / sp->res[0], fmax = sp->pix / sp->res[1];
+	k_moments1<<<1, 256, 0, ctx->stream>>>(ctx->spect.as<float>(), cx * T, ctx->moments1.as<double>());
+	k_full_spectrum<<<T, 256, 0, ctx->stream>>>(ctx->spect.as<float>(), cx, T, sp->cc, dmin, dmax, az, add,
+		fmin, fmax, ctx->moments1.as<double>(), ctx->fullsp.as<float>());
+	HIPCHK(ctx, hipGetLastError());
+	HIPCHK(ctx, hipMemcpyAsync(host_out, ctx->fullsp.p, sizeof(float) * (size_t)T * T, hipMemcpyDeviceToHost,
+		ctx->stream));
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	return GCTF_OK;
+}
+
+// ---- batched path ----------------------------------------------------------
+namespace {
+
+int record_batch(gctf_ctx* c, const RescalePlan& rp, SearchPlan& sp, TileTables& tt, const void* imgs,
+	int dtype, int n, long long stride)
+{
+	const int T = c->prm.tile_size;
+	const TileGrid g = make_grid(rp.nnx, rp.nny, T);
+	const int C = std::max(1, std::min(c->chunk, n));
+	k_state_init<<<(n + 63) / 64, 64, 0, c->stream>>>(c->states.as<SearchState>(), n, c->prm.ext_phase_deg,
+		std::max(c->prm.ext_phase_range_deg, 0.0f));
+	const size_t esz = dtype == GCTF_DTYPE_F16 ? 2 : 4;
+	for (int i0 = 0; i0 < n; i0 += C) {
+		const int m = std::min(C, n - i0);
+		const char* src = (const char*)imgs + (size_t)i0 * stride * esz;
+		if (rp.active) {
+			float* pad = c->padc.as<float>();
+			const long long ps = (long long)rp.pitch * rp.nny;
+			CHK(launch_rescale(c, rp, src, dtype, stride, m, pad, ps));
+			CHK(launch_spectrum<float>(c, g, tt, pad, rp.pitch, ps, m, i0));
+		} else if (dtype == GCTF_DTYPE_F16) {
+			CHK(launch_spectrum<_Float16>(c, g, tt, (const _Float16*)src, rp.nx, stride, m, i0));
+		} else {
+			CHK(launch_spectrum<float>(c, g, tt, (const float*)src, rp.nx, stride, m, i0));
+		}
+	}
+	CHK(launch_post(c, tt, n));
+	CHK(launch_1d(c, sp, tt, n, 0, 0.f, 0.f));
+	CHK(launch_2d(c, sp, n, NX_BEGIN_DO2D, std::max(c->prm.ext_phase_range_deg, 0.0f)));
+	return GCTF_OK;
+}
+
+}  // namespace
+
+int gctf_estimate_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny,
+	long long stride, float pix, gctf_result* out)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	if (!dev_imgs || !out || n < 1) return fail(ctx, GCTF_ERR_ARG, "null input/output or n < 1");
+	if (dtype != GCTF_DTYPE_F32 && dtype != GCTF_DTYPE_F16) return fail(ctx, GCTF_ERR_ARG, "dtype");
+	if (stride < (long long)nx * ny) return fail(ctx, GCTF_ERR_ARG, "stride smaller than one image");
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	RescalePlan rp;
+	CHK(make_rescale_plan(ctx, nx, ny, pix, rp));
+	const int T = ctx->prm.tile_size;
+	if (rp.nnx < T || rp.nny < T) return fail(ctx, GCTF_ERR_ARG, "image smaller than one tile");
+	const TileGrid g = make_grid(rp.nnx, rp.nny, T);
+	if (g.ntx < 1 || g.nty < 1) return fail(ctx, GCTF_ERR_ARG, "no tile fits the image");
+	SearchPlan* sp;
+	TileTables* tt;
+	CHK(get_search_plan(ctx, ctx->prm.kv, ctx->prm.cs_mm, ctx->prm.amp_contrast, rp.pixn, T, &sp));
+	CHK(get_tile_tables(ctx, T, &tt));
+	const int C = std::max(1, std::min(ctx->chunk, n));
+	CHK(ensure_image_buffers(ctx, n, T, sp->ring.n));
+	CHK(ensure_spectrum_buffers(ctx, g, T, C));
+	if (rp.active) {
+		CHK(ensure_rescale_buffers(ctx, rp, C));
+		CHK(ensure_buf(ctx, ctx->padc, sizeof(float) * (size_t)rp.pitch * rp.nny * C));
+	}
+	if (!ctx->use_graph) {
+		CHK(record_batch(ctx, rp, *sp, *tt, dev_imgs, dtype, n, stride));
+		return read_results(ctx, n, out);
+	}
+	char key[256];
+	snprintf(key, sizeof key, "%p/%d/%d/%d/%d/%lld/%a/%d/%d", dev_imgs, dtype, n, nx, ny, stride, pix, C,
+		ctx->tile_splits);
+	GraphEntry& ge = ctx->graphs[key];
+	if (!ge.exec || ge.gen != ctx->gen) {
+		if (ge.exec) { (void)hipGraphExecDestroy(ge.exec); ge.exec = nullptr; }
+		hipGraph_t graph = nullptr;
+		HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+		int r = record_batch(ctx, rp, *sp, *tt, dev_imgs, dtype, n, stride);
+		hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+		if (r != GCTF_OK) { if (graph) (void)hipGraphDestroy(graph); return r; }
+		HIPCHK(ctx, e);
+		hipError_t ei = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+		(void)hipGraphDestroy(graph);
+		HIPCHK(ctx, ei);
+		ge.gen = ctx->gen;
+	}
+	HIPCHK(ctx, hipGraphLaunch(ge.exec, ctx->stream));
+	return read_results(ctx, n, out);
+}
+
+}  // extern "C"

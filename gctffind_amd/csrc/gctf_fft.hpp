@@ -1,0 +1,272 @@
+// gctf_fft.hpp -- LDS-resident complex FFTs for gfx950.
+//
+// Power-of-two lengths use a Stockham autosort (radix 8, last pass radix 2
+// or 4) whose butterflies live in registers; one workgroup-wide barrier
+// separates the LDS reads of a pass from its writes, so one LDS buffer of N
+// complex values suffices.  Other lengths use Bluestein's chirp-z algorithm
+// on a power-of-two length M >= 2n-1.  Both are unnormalised; the forward
+// transform uses exp(-2 pi i k n / N).
+//
+// Twiddle tables (tw[e] = exp(-2 pi i e / N), e < N) and Bluestein chirps
+// are computed on the host in double and rounded once to float.
+//
+// All code here is compiled with -ffp-contract=off; fused multiply-adds
+// are written out with fmaf where they are wanted.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace gctf {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+	return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 mul_wi(float2 a)
+{
+	return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft_small(float2* u);
+
+template <>
+__device__ __forceinline__ void dft_small<2, false>(float2* u)
+{
+	float2 a = u[0], b = u[1];
+	u[0] = cadd(a, b); u[1] = csub(a, b);
+}
+template <>
+__device__ __forceinline__ void dft_small<2, true>(float2* u)
+{
+	float2 a = u[0], b = u[1];
+	u[0] = cadd(a, b); u[1] = csub(a, b);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3)
+{
+	float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+	float2 t2 = cadd(a1, a3), t3 = mul_wi<INV>(csub(a1, a3));
+	a0 = cadd(t0, t2); a2 = csub(t0, t2);
+	a1 = cadd(t1, t3); a3 = csub(t1, t3);
+}
+
+template <>
+__device__ __forceinline__ void dft_small<4, false>(float2* u) { dft4<false>(u[0], u[1], u[2], u[3]); }
+template <>
+__device__ __forceinline__ void dft_small<4, true>(float2* u) { dft4<true>(u[0], u[1], u[2], u[3]); }
+
+template <bool INV>
+__device__ __forceinline__ void dft8(float2* u)
+{
+	const float c = 0.70710678118654752f;
+	float2 e0 = u[0], e1 = u[2], e2 = u[4], e3 = u[6];
+	float2 o0 = u[1], o1 = u[3], o2 = u[5], o3 = u[7];
+	dft4<INV>(e0, e1, e2, e3);
+	dft4<INV>(o0, o1, o2, o3);
+	// o_k *= W8^k
+	float2 w1, w3;
+	if (INV) {
+		w1 = make_float2(c * (o1.x - o1.y), c * (o1.x + o1.y));
+		w3 = make_float2(-c * (o3.x + o3.y), c * (o3.x - o3.y));
+	} else {
+		w1 = make_float2(c * (o1.x + o1.y), c * (o1.y - o1.x));
+		w3 = make_float2(c * (o3.y - o3.x), -c * (o3.x + o3.y));
+	}
+	float2 w2 = mul_wi<INV>(o2);
+	u[0] = cadd(e0, o0); u[4] = csub(e0, o0);
+	u[1] = cadd(e1, w1); u[5] = csub(e1, w1);
+	u[2] = cadd(e2, w2); u[6] = csub(e2, w2);
+	u[3] = cadd(e3, w3); u[7] = csub(e3, w3);
+}
+
+template <>
+__device__ __forceinline__ void dft_small<8, false>(float2* u) { dft8<false>(u); }
+template <>
+__device__ __forceinline__ void dft_small<8, true>(float2* u) { dft8<true>(u); }
+
+// One Stockham pass of radix R over N points; L = product of the radices of
+// the previous passes.  NT threads (tid in [0, NT)) share the buffer.
+template <int N, int R, int L, int NT, bool INV>
+__device__ __forceinline__ void fft_pass(float2* buf, const float2* __restrict__ tw, int tid)
+{
+	constexpr int T = N / R;
+	constexpr int BPT = (T + NT - 1) / NT;
+	float2 u[BPT][R];
+#pragma unroll
+	for (int b = 0; b < BPT; ++b) {
+		const int j = tid + b * NT;
+		if (T % NT == 0 || j < T) {
+#pragma unroll
+			for (int r = 0; r < R; ++r) u[b][r] = buf[j + r * T];
+			if constexpr (L > 1) {
+				const int k = j & (L - 1);
+				constexpr int S = N / (L * R);
+#pragma unroll
+				for (int r = 1; r < R; ++r) {
+					float2 w = tw[(r * k) * S];
+					if (INV) w.y = -w.y;
+					u[b][r] = cmul(u[b][r], w);
+				}
+			}
+			dft_small<R, INV>(u[b]);
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int b = 0; b < BPT; ++b) {
+		const int j = tid + b * NT;
+		if (T % NT == 0 || j < T) {
+			const int k = j & (L - 1);
+			const int base = (j - k) * R + k;
+#pragma unroll
+			for (int r = 0; r < R; ++r) buf[base + r * L] = u[b][r];
+		}
+	}
+	__syncthreads();
+}
+
+template <int N, int L, int NT, bool INV>
+__device__ __forceinline__ void fft_run(float2* buf, const float2* __restrict__ tw, int tid)
+{
+	if constexpr (L < N) {
+		constexpr int rem = N / L;
+		constexpr int R = rem >= 8 ? 8 : rem;
+		fft_pass<N, R, L, NT, INV>(buf, tw, tid);
+		fft_run<N, L * R, NT, INV>(buf, tw, tid);
+	}
+}
+
+// In-place power-of-two FFT of buf[0..N) in LDS.  Every thread of the
+// workgroup must call it (it contains __syncthreads); the data must be in
+// buf and visible (barrier) before the call; results are visible after.
+template <int N, int NT, bool INV>
+__device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw, int tid)
+{
+	fft_run<N, 1, NT, INV>(buf, tw, tid);
+}
+
+// ---- runtime-length power-of-two FFT (lengths up to MAXN) ----------------
+template <int R, int NT, int MAXN, bool INV>
+__device__ __forceinline__ void fft_pass_rt(float2* buf, const float2* __restrict__ tw, int tid,
+	int L, int T, int S)
+{
+	constexpr int BPT = (MAXN / R + NT - 1) / NT;
+	float2 u[BPT][R];
+#pragma unroll
+	for (int b = 0; b < BPT; ++b) {
+		const int j = tid + b * NT;
+		if (j < T) {
+#pragma unroll
+			for (int r = 0; r < R; ++r) u[b][r] = buf[j + r * T];
+			if (L > 1) {
+				const int k = j & (L - 1);
+#pragma unroll
+				for (int r = 1; r < R; ++r) {
+					float2 w = tw[(r * k) * S];
+					if (INV) w.y = -w.y;
+					u[b][r] = cmul(u[b][r], w);
+				}
+			}
+			dft_small<R, INV>(u[b]);
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int b = 0; b < BPT; ++b) {
+		const int j = tid + b * NT;
+		if (j < T) {
+			const int k = j & (L - 1);
+			const int base = (j - k) * R + k;
+#pragma unroll
+			for (int r = 0; r < R; ++r) buf[base + r * L] = u[b][r];
+		}
+	}
+	__syncthreads();
+}
+
+// In-place power-of-two FFT of runtime length n <= MAXN; same contract as
+// fft_lds.
+template <int NT, int MAXN, bool INV>
+__device__ __forceinline__ void fft_lds_rt(float2* buf, int n, const float2* __restrict__ tw, int tid)
+{
+	int L = 1;
+	while (L < n) {
+		const int rem = n / L;
+		if (rem >= 8) {
+			fft_pass_rt<8, NT, MAXN, INV>(buf, tw, tid, L, n / 8, n / (L * 8));
+			L *= 8;
+		} else if (rem == 4) {
+			fft_pass_rt<4, NT, MAXN, INV>(buf, tw, tid, L, n / 4, 1);
+			L *= 4;
+		} else {
+			fft_pass_rt<2, NT, MAXN, INV>(buf, tw, tid, L, n / 2, 1);
+			L *= 2;
+		}
+	}
+}
+
+// Bluestein DFT of length n held in buf[0..n); buf has m entries (m a power
+// of two >= 2n-1).  chirp[k] = exp(-pi i (k^2 mod 2n) / n); bhat = FFT_m of
+// b[j] = conj(chirp[|j|]) (wrapped).  twm: the m-point twiddle table.
+// The inverse uses conj(chirp) and bhat_inv[j] = conj(bhat[(m-j) mod m]).
+template <int NT, int MAXN, bool INV>
+__device__ __forceinline__ void bluestein_lds(float2* buf, int n, int m, const float2* __restrict__ chirp,
+	const float2* __restrict__ bhat, const float2* __restrict__ twm, int tid)
+{
+	for (int k = tid; k < m; k += NT) {
+		if (k < n) {
+			float2 w = chirp[k];
+			if (INV) w.y = -w.y;
+			buf[k] = cmul(buf[k], w);
+		} else {
+			buf[k] = make_float2(0.f, 0.f);
+		}
+	}
+	__syncthreads();
+	fft_lds_rt<NT, MAXN, false>(buf, m, twm, tid);
+	for (int k = tid; k < m; k += NT) {
+		float2 b;
+		if (INV) {
+			b = bhat[(m - k) & (m - 1)];
+			b.y = -b.y;
+		} else {
+			b = bhat[k];
+		}
+		buf[k] = cmul(buf[k], b);
+	}
+	__syncthreads();
+	fft_lds_rt<NT, MAXN, true>(buf, m, twm, tid);
+	const float inv_m = 1.0f / (float)m;
+	for (int k = tid; k < n; k += NT) {
+		float2 w = chirp[k];
+		if (INV) w.y = -w.y;
+		float2 v = cmul(buf[k], w);
+		buf[k] = make_float2(v.x * inv_m, v.y * inv_m);
+	}
+	__syncthreads();
+}
+
+// A 1D transform of arbitrary length n in LDS, as planned on the host.
+struct Fft1dPlan {
+	int n;                   // transform length
+	int m;                   // power-of-two working length (n when n is a power of two)
+	const float2* tw;        // m-point twiddles
+	const float2* chirp;     // n entries (Bluestein only, else null)
+	const float2* bhat;      // m entries (Bluestein only, else null)
+};
+
+template <int NT, int MAXN, bool INV>
+__device__ __forceinline__ void fft_any_lds(float2* buf, const Fft1dPlan& p, int tid)
+{
+	if (p.chirp == nullptr) fft_lds_rt<NT, MAXN, INV>(buf, p.n, p.tw, tid);
+	else bluestein_lds<NT, MAXN, INV>(buf, p.n, p.m, p.chirp, p.bhat, p.tw, tid);
+}
+
+}  // namespace gctf

@@ -1,0 +1,644 @@
+// k_search.hpp -- correlation-driven defocus search on the device.
+//
+// The candidate sequences, step counts, ranges and acceptance rules are the
+// reference's (CFindDefocus1D.cpp, CFindCtf1D.cpp, CFindDefocus2D.cpp),
+// evaluated in IEEE single with no contraction (this library is built with
+// -ffp-contract=off), so every candidate value is bit-identical to a host
+// evaluation of the same float expressions.  The correlation sums run in
+// FP32 per thread and FP64 across threads and pixel chunks.
+#pragma once
+#include "gctf_internal.hpp"
+
+namespace gctf {
+
+constexpr float kD2R2D = 0.01745329f;    // CFindDefocus2D.cpp:11
+constexpr float kD2R1D = 0.0174533f;     // CFindDefocus1D.cpp:11
+constexpr float kPi1D = 3.141592654f;    // GCalcCTF1D.cu:30
+constexpr float kPi2D = 3.1415926f;      // GCalcCTF2D.cu:35
+
+// |sin(pi u)|: u reduced to f in [0, 1), t = min(f, 1-f) in [0, 1/2],
+// sin(pi t) by its Taylor series to x^13 (|error| < 2e-9 on [0, pi/2]).
+__device__ __forceinline__ float abs_sinpi(float u)
+{
+	float f = u - floorf(u);
+	float t = fminf(f, 1.0f - f);
+	float x = t * 3.14159265358979f;
+	float x2 = x * x;
+	float p = fmaf(x2, -1.6059043836821614e-10f, 2.5052108385441720e-08f);
+	p = fmaf(x2, p, -2.7557319223985893e-06f);
+	p = fmaf(x2, p, 1.9841269841269841e-04f);
+	p = fmaf(x2, p, -8.3333333333333333e-03f);
+	p = fmaf(x2, p, 1.6666666666666667e-01f);
+	p = fmaf(-x2, p, 1.0f);
+	return x * p;
+}
+
+// ---------------------------------------------------------------------------
+// 1D search (CFindDefocus1D::mBrutalForceSearch, CFindDefocus1D.cpp:76-119;
+// CFindCtf1D::mFindDefocus / mRefineDefocus, CFindCtf1D.cpp:73-111).
+// One workgroup per micrograph.
+// ---------------------------------------------------------------------------
+struct Search1DParams {
+	CtfConsts c;
+	int cmpx;
+	float lo, hi;            // bins x with lo <= x < hi (Fourier pixels)
+	int mode;                // 0: Do1D, 1: Refine1D(init_df, range)
+	float init_df, range;    // Refine1D arguments
+};
+
+__device__ __forceinline__ float cc1d_eval(const Search1DParams& p, const float* rad,
+	const double* w1d, float df_a, float ph_deg)
+{
+	float ext = ph_deg * kD2R1D;                    // CFindDefocus1D.cpp:124
+	float dfp = df_a / p.c.pix;                     // :125
+	float add = p.c.amp_phase + ext;                // GCalcCTF1D.cu:63
+	const double wl = p.c.wl, cs = p.c.cs;
+	double s = 0.0, s1 = 0.0, s2 = 0.0;
+	for (int x = 0; x < p.cmpx; ++x) {
+		if (!((float)x >= p.lo && (float)x < p.hi)) continue;
+		double f2 = (double)x * 0.5 / (double)(p.cmpx - 1);
+		f2 *= f2;
+		double chi = (double)add + (double)kPi1D * wl * f2 * ((double)dfp - 0.5 * wl * wl * f2 * cs);
+		double c = (fabs(sin(chi)) - 0.5) * w1d[x];
+		double r = (double)rad[x];
+		s += c * r; s1 += c * c; s2 += r * r;
+	}
+	if (s1 > 0 && s2 > 0) return (float)(s / sqrt(s1 * s2));
+	return 0.0f;
+}
+
+// Brute force over (df, phase); returns the first strict maximum.
+__device__ void search1d_brute(const Search1DParams& p, const float* rad, const double* w1d,
+	float dfr0, float dfr1, float phr0, float phr1, float* out3, float* s_cc, int* s_i)
+{
+	int ndf = 501;
+	float dfrange = dfr1 - dfr0;
+	float dfstep = dfrange / (float)(ndf - 1);
+	if (dfstep < 10) dfstep = 10.0f;
+	ndf = (int)(dfrange / dfstep) / 2 * 2 + 1;
+	int nps = 37;
+	float psrange = phr1 - phr0;
+	float psstep = psrange / (float)(nps - 1);
+	if (psstep == 0) nps = 1;
+	else nps = (int)(psrange / psstep) / 2 * 2 + 1;
+	const int npts = ndf * nps;
+	float best = (float)-1e20;
+	int bi = -1;
+	for (int i = threadIdx.x; i < npts; i += blockDim.x) {
+		int ifo = i % ndf, iph = i / ndf;
+		float df = dfr0 + (float)ifo * dfstep;
+		float ph = phr0 + (float)iph * psstep;
+		if (ph < 0) ph = 0.0f;
+		else if (ph > 150.0f) ph = 150.0f;
+		float cc = cc1d_eval(p, rad, w1d, df, ph);
+		if (cc > best) { best = cc; bi = i; }
+	}
+	s_cc[threadIdx.x] = best;
+	s_i[threadIdx.x] = bi;
+	__syncthreads();
+	for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+		if (threadIdx.x < s) {
+			float c2 = s_cc[threadIdx.x + s];
+			int i2 = s_i[threadIdx.x + s];
+			float c1 = s_cc[threadIdx.x];
+			int i1 = s_i[threadIdx.x];
+			if (i2 >= 0 && (i1 < 0 || c2 > c1 || (c2 == c1 && i2 < i1))) {
+				s_cc[threadIdx.x] = c2; s_i[threadIdx.x] = i2;
+			}
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		int i = s_i[0];
+		float r0 = 0.f, r1 = 0.f, r2 = (float)-1e20;
+		if (i >= 0) {
+			int ifo = i % ndf, iph = i / ndf;
+			r0 = dfr0 + (float)ifo * dfstep;
+			float ph = phr0 + (float)iph * psstep;
+			if (ph < 0) ph = 0.0f;
+			else if (ph > 150.0f) ph = 150.0f;
+			r1 = ph; r2 = s_cc[0];
+		}
+		out3[0] = r0; out3[1] = r1; out3[2] = r2;
+	}
+	__syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_search1d(Search1DParams p, const float* __restrict__ radial,
+	const double* __restrict__ w1d, SearchState* __restrict__ st)
+{
+	__shared__ float s_cc[256];
+	__shared__ int s_i[256];
+	__shared__ float res[3];
+	SearchState* S = st + blockIdx.x;
+	const float* rad = radial + (long long)blockIdx.x * p.cmpx;
+	const float pix = p.c.pix;
+	float df_min, ext_phase;
+	if (p.mode == 0) {
+		// CFindCtf1D::mFindDefocus
+		float p2 = pix * pix;
+		float dfr0 = 3000.0f * p2, dfr1 = 30000.0f * p2;
+		float phr0 = fmaxf(S->ext_phase - S->phase_range / 2, 0.0f);
+		float phr1 = fminf(phr0 + S->phase_range, 150.0f);
+		search1d_brute(p, rad, w1d, dfr0, dfr1, phr0, phr1, res, s_cc, s_i);
+		df_min = res[0];
+		ext_phase = res[1];
+	} else {
+		// CFindCtf1D::Refine1D
+		df_min = p.init_df;
+		ext_phase = S->ext_phase;
+	}
+	// CFindCtf1D::mRefineDefocus
+	float range = p.mode == 0 ? fmaxf(0.3f * df_min, 3000.0f) : p.range;
+	float mindf = 3000.0f * pix * pix;
+	float dfr0 = fmaxf(df_min - range / 2, mindf);
+	float dfr1 = dfr0 + range;
+	float prange = 0.2f * S->phase_range;
+	float phr0 = fmaxf(ext_phase - prange / 2, 0.0f);
+	float phr1 = fminf(0.0f + prange, 150.0f);
+	search1d_brute(p, rad, w1d, dfr0, dfr1, phr0, phr1, res, s_cc, s_i);
+	if (threadIdx.x == 0) {
+		S->ext_phase = res[1];
+		S->df_min = res[0];
+		S->df_max = res[0];
+		S->score = res[2];
+	}
+}
+
+// ---------------------------------------------------------------------------
+// 2D search (CFindDefocus2D.cpp:100-380).
+// ---------------------------------------------------------------------------
+
+// Candidate c of the pending stage -> (dfmean, ast, ang, phase); false for
+// phase candidates outside the window (CFindDefocus2D.cpp:347).
+__device__ __forceinline__ bool stage_candidate(const SearchState& s, int c, float& dfm, float& ast,
+	float& ang, float& ph)
+{
+	dfm = s.dfmean; ast = s.ast; ang = s.ang; ph = s.phase;
+	switch (s.st_kind) {
+	case ST_ASTIG: {
+		int j = c / s.st_nast, i = c % s.st_nast;
+		ang = s.st_b0 + (float)j * s.st_bstep;
+		ast = s.st_a0 + (float)i * s.st_astep;
+		return true;
+	}
+	case ST_ASTMAG: ast = s.st_a0 + (float)c * s.st_astep; return true;
+	case ST_ASTANG: ang = s.st_a0 + (float)c * s.st_astep; return true;
+	case ST_DFMEAN: dfm = s.st_a0 + (float)c * s.st_astep; return true;
+	case ST_PHASE:
+		ph = s.phase + (float)(c - s.st_n / 2) * s.st_astep;
+		return !(ph < s.st_lo || ph > s.st_hi);
+	default: return true;
+	}
+}
+
+// Per-candidate CTF parameters (CFindDefocus2D::mCorrelate :363-380 and
+// GCalcCTF2D::DoIt :99-114) in the form the evaluator uses:
+// u = chi / pi = ph0 + a * (mean + sig * g) - b with g = cos(2 (theta - az)).
+struct CandCtf {
+	float mean, sig, ca, sa, ph0;
+};
+
+__device__ __forceinline__ CandCtf cand_ctf(const CtfConsts& c, float dfm, float ast, float ang, float ph)
+{
+	float ph_rad = ph * kD2R2D;
+	float az = ang * kD2R2D;
+	float dmin = dfm * (1.0f - ast) / c.pix;
+	float dmax = dfm * (1.0f + ast) / c.pix;
+	CandCtf o;
+	o.mean = 0.5f * (dmin + dmax);
+	o.sig = 0.5f * (dmax - dmin);
+	float add = c.amp_phase + ph_rad;
+	double a2 = 2.0 * (double)az;
+	o.ca = (float)cos(a2);
+	o.sa = (float)sin(a2);
+	o.ph0 = (float)((double)add / 3.14159265358979323846);
+	return o;
+}
+
+// Candidate evaluation.  Grid (ceil(max candidates / CPB), kPixChunks,
+// images); 256 threads.  part[image][cand][chunk] = {sum c*S, sum c*c}.
+template <int CPB>
+__global__ __launch_bounds__(256) void k_cc2d_eval(CtfConsts cc, RingTables ring,
+	const float* __restrict__ ring_spec, const SearchState* __restrict__ st,
+	double2* __restrict__ part)
+{
+	__shared__ CandCtf cand[CPB];
+	__shared__ int cvalid[CPB];
+	__shared__ double red[2][CPB][4];
+	const SearchState& S = st[blockIdx.z];
+	if (S.done || S.st_kind == ST_NONE) return;
+	const int n = S.st_n;
+	const int c0 = blockIdx.x * CPB;
+	if (c0 >= n) return;
+	if (threadIdx.x < CPB) {
+		int c = c0 + threadIdx.x;
+		float dfm, ast, ang, ph;
+		bool ok = c < n && stage_candidate(S, c, dfm, ast, ang, ph);
+		cvalid[threadIdx.x] = ok;
+		cand[threadIdx.x] = ok ? cand_ctf(cc, dfm, ast, ang, ph) : CandCtf{0.f, 0.f, 0.f, 0.f, 0.f};
+	}
+	__syncthreads();
+	const int chunk = blockIdx.y;
+	const int per = (ring.n + kPixChunks - 1) / kPixChunks;
+	const int k0 = chunk * per, k1 = min(ring.n, k0 + per);
+	const float* sp = ring_spec + (long long)blockIdx.z * ring.n;
+	float acc_cs[CPB], acc_cc[CPB];
+#pragma unroll
+	for (int j = 0; j < CPB; ++j) { acc_cs[j] = 0.f; acc_cc[j] = 0.f; }
+	CandCtf cd[CPB];
+#pragma unroll
+	for (int j = 0; j < CPB; ++j) cd[j] = cand[j];
+	for (int k = k0 + threadIdx.x; k < k1; k += 256) {
+		const float a = ring.a[k], b = ring.b[k], c2 = ring.c2[k], s2 = ring.s2[k];
+		const float w = ring.w[k], sv = sp[k];
+#pragma unroll
+		for (int j = 0; j < CPB; ++j) {
+			float g = fmaf(c2, cd[j].ca, s2 * cd[j].sa);
+			float df = fmaf(cd[j].sig, g, cd[j].mean);
+			float u = fmaf(a, df, cd[j].ph0 - b);
+			float cv = (abs_sinpi(u) - 0.5f) * w;
+			acc_cs[j] = fmaf(cv, sv, acc_cs[j]);
+			acc_cc[j] = fmaf(cv, cv, acc_cc[j]);
+		}
+	}
+	// reduce: wave shuffles in double, then across the 4 waves in LDS
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+	for (int j = 0; j < CPB; ++j) {
+		double x = acc_cs[j], y = acc_cc[j];
+		for (int o = 32; o > 0; o >>= 1) {
+			x += __shfl_xor(x, o);
+			y += __shfl_xor(y, o);
+		}
+		if (lane == 0) { red[0][j][wave] = x; red[1][j][wave] = y; }
+	}
+	__syncthreads();
+	if (threadIdx.x < CPB) {
+		const int j = threadIdx.x, c = c0 + j;
+		if (c < n) {
+			double x = red[0][j][0] + red[0][j][1] + red[0][j][2] + red[0][j][3];
+			double y = red[1][j][0] + red[1][j][1] + red[1][j][2] + red[1][j][3];
+			part[((long long)blockIdx.z * kMaxCands + c) * kPixChunks + chunk] = make_double2(x, y);
+		}
+	}
+}
+
+// Gather the ring pixels of each spectrum and their FP64 energy.
+__global__ __launch_bounds__(1024) void k_ring_gather(const float* __restrict__ spect, int npix,
+	RingTables ring, float* __restrict__ ring_spec, double* __restrict__ ring_ss)
+{
+	__shared__ double red[1024];
+	const float* in = spect + (long long)blockIdx.x * npix;
+	float* out = ring_spec + (long long)blockIdx.x * ring.n;
+	double acc = 0.0;
+	for (int k = threadIdx.x; k < ring.n; k += 1024) {
+		float v = in[ring.idx[k]];
+		out[k] = v;
+		acc += (double)v * v;
+	}
+	red[threadIdx.x] = acc;
+	__syncthreads();
+	for (int s = 512; s > 0; s >>= 1) {
+		if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) ring_ss[blockIdx.x] = red[0];
+}
+
+// ---- controller -----------------------------------------------------------
+// Stage preparation, exactly the float code of the reference's methods.
+
+__device__ __forceinline__ void prep_none(SearchState& s) { s.st_kind = ST_NONE; s.st_n = 0; }
+
+// mFindAstig (CFindDefocus2D.cpp:185-200)
+__device__ void prep_astig(SearchState& s)
+{
+	const float tiny = (float)1e-20;
+	float astr = s.astrange[1] - s.astrange[0];
+	float angr = s.angrange[1] - s.angrange[0];
+	if (astr < tiny && angr < tiny) { prep_none(s); return; }
+	int nast = 51, nang = 51;
+	float aststep = astr / (float)nast;
+	float angstep = angr / (float)nang;
+	if ((double)aststep < 1e-5) nast = 1;
+	if ((double)angstep < 1e-5) nang = 1;
+	if (nast == 1 && nang == 1) { prep_none(s); return; }
+	s.st_kind = ST_ASTIG;
+	s.st_n = nast * nang;
+	s.st_nast = nast;
+	s.st_a0 = s.astrange[0]; s.st_astep = aststep;
+	s.st_b0 = s.angrange[0]; s.st_bstep = angstep;
+}
+
+// mRefineAstMag / mRefineAstAng (CFindDefocus2D.cpp:224-286)
+__device__ void prep_line(SearchState& s, int kind, float cur, const float* lim, float range,
+	int n0, float minstep)
+{
+	const float tiny = (float)1e-20;
+	if (range < tiny) { prep_none(s); return; }
+	float mn = fmaxf(cur - range * 0.5f, lim[0]);
+	float mx = fminf(cur + range * 0.5f, lim[1]);
+	int n = n0;
+	float step = (mx - mn) / (float)(n - 1);
+	if (step < minstep) step = minstep;
+	n = (int)((mx - mn) / step) / 2 * 2 + 1;
+	if (n <= 1) { prep_none(s); return; }
+	s.st_kind = kind; s.st_n = n; s.st_a0 = mn; s.st_astep = step;
+}
+
+// mRefineDfMean (CFindDefocus2D.cpp:288-322)
+__device__ void prep_dfmean(SearchState& s, float range)
+{
+	const float tiny = (float)1e-30;
+	if (range < tiny) { prep_none(s); return; }
+	int n = 31;
+	float mn = fmaxf(s.dfmean - range * 0.5f, s.dfrange[0]);
+	float mx = fminf(s.dfmean + range * 0.5f, s.dfrange[1]);
+	float step = (mx - mn) / (float)n;
+	if (step < 20) step = 20.0f;
+	n = (int)((mx - mn) / step) / 2 * 2 + 1;
+	if (n <= 1) { prep_none(s); return; }
+	s.st_kind = ST_DFMEAN; s.st_n = n; s.st_a0 = mn; s.st_astep = step;
+	s.st_old = s.dfmean;
+}
+
+// mRefinePhase (CFindDefocus2D.cpp:324-361)
+__device__ void prep_phase(SearchState& s, float range)
+{
+	const float tiny = (float)1e-30;
+	if (range < tiny) { prep_none(s); return; }
+	int n = 61;
+	float step = range / (float)n;
+	if (step < 0.03f) { prep_none(s); return; }
+	if (step > 1.0f) {
+		step = 1.0f;
+		n = (int)(range / step);
+		n = n / 2 * 2 + 1;
+	}
+	float mnp = s.phrange[0] - s.phrange[1] * 0.5f;
+	float mxp = s.phrange[0] + s.phrange[1] * 0.5f;
+	if (mnp < 0) mnp = 0.0f;
+	if (mxp > 150) mxp = 150.0f;
+	if (n > kMaxCands) n = kMaxCands;
+	s.st_kind = ST_PHASE; s.st_n = n; s.st_astep = step; s.st_lo = mnp; s.st_hi = mxp;
+}
+
+__device__ __forceinline__ float iter_scale(int i)
+{
+	return 1.0f - (float)i * 0.5f / (float)20;   // CFindDefocus2D.cpp:167
+}
+
+// Stage program of mIterate; `next` names what the controller prepares.
+enum NextStage : int {
+	NX_BEGIN_DO2D = 0,   // CFindDefocus2D::DoIt setup, then INIT
+	NX_BEGIN_REFINE = 1, // CFindDefocus2D::Refine setup, then INIT
+	NX_ASTIG = 2,        // consume INIT, prepare ASTIG
+	NX_ASTMAG = 3,       // consume ASTIG / PHASE, prepare ASTMAG(iter)
+	NX_ASTANG = 4,       // consume ASTMAG, prepare ASTANG
+	NX_DFMEAN = 5,       // consume ASTANG, prepare DFMEAN
+	NX_PHASE = 6,        // consume DFMEAN, prepare PHASE
+	NX_FINISH = 7,       // consume PHASE of the last iteration
+};
+
+struct RefineArgs {
+	float dfmean[2], ast[2], ang[2], phase[2];
+};
+
+// One workgroup (256 threads) per micrograph: consume the CCs of the stage
+// just evaluated (first strict maximum, `<= continue` order), update the
+// state as the reference method does, and prepare the next stage.
+__global__ __launch_bounds__(256) void k_s2d_step(SearchState* __restrict__ st,
+	const double2* __restrict__ part, const double* __restrict__ ring_ss, int next,
+	const RefineArgs* __restrict__ refine)
+{
+	__shared__ float s_cc[256];
+	__shared__ int s_i[256];
+	SearchState* S = st + blockIdx.x;
+	const int tid = threadIdx.x;
+	if (next == NX_BEGIN_DO2D || next == NX_BEGIN_REFINE) {
+		if (tid == 0) {
+			SearchState s = *S;
+			if (next == NX_BEGIN_DO2D) {
+				// CFindCtf2D::Do2D (CFindCtf2D.cpp:44-53) + DoIt (:110-124)
+				float dfmean = (s.df_min + s.df_max) * 0.5f;
+				s.dfmean = dfmean; s.ast = 0.0f; s.ang = 0.0f;
+				s.phrange[0] = s.ext_phase;
+				s.phrange[1] = s.phase_range;
+				s.dfrange[0] = s.dfmean * 0.9f; s.dfrange[1] = s.dfmean * 1.1f;
+				s.astrange[0] = 0.0f; s.astrange[1] = 0.06f;
+				s.angrange[0] = -90.0f; s.angrange[1] = 90.0f;
+			} else {
+				// CFindCtf2D::Refine (CFindCtf2D.cpp:55-68) + Refine (:126-151)
+				const RefineArgs& r = refine[blockIdx.x];
+				s.dfmean = r.dfmean[0]; s.ast = r.ast[0]; s.ang = r.ang[0];
+				s.phrange[0] = r.phase[0];
+				float h = 0.5f * r.dfmean[1];
+				s.dfrange[0] = fmaxf(s.dfmean - h, 3000.0f);
+				s.dfrange[1] = s.dfmean + h;
+				h = 0.5f * r.ast[1];
+				s.astrange[0] = fmaxf(s.ast - h, 0.0f);
+				s.astrange[1] = fminf(s.ast + h, 0.02f);
+				h = 0.5f * r.ang[1];
+				s.angrange[0] = fmaxf(s.ang - h, -90.0f);
+				s.angrange[1] = fminf(s.ang + h, 90.0f);
+				s.phrange[1] = r.phase[1];
+			}
+			s.done = 0; s.iter = 0;
+			s.st_kind = ST_INIT; s.st_n = 1;
+			*S = s;
+		}
+		return;
+	}
+	// ---- argmax of the evaluated stage
+	const SearchState s0 = *S;
+	const bool ran = !s0.done && s0.st_kind != ST_NONE && s0.st_n > 0;
+	float best = (float)-1e20;
+	int bi = -1;
+	if (ran) {
+		const double ss = ring_ss[blockIdx.x];
+		const double2* pp = part + (long long)blockIdx.x * kMaxCands * kPixChunks;
+		for (int c = tid; c < s0.st_n; c += 256) {
+			float dfm, ast, ang, ph;
+			if (!stage_candidate(s0, c, dfm, ast, ang, ph)) continue;
+			double x = 0.0, y = 0.0;
+			for (int k = 0; k < kPixChunks; ++k) {
+				double2 v = pp[(long long)c * kPixChunks + k];
+				x += v.x; y += v.y;
+			}
+			double d = sqrt(y * ss);
+			float cc = d == 0.0 ? 0.0f : (float)(x / d);
+			if (cc > best) { best = cc; bi = c; }
+		}
+	}
+	s_cc[tid] = best; s_i[tid] = bi;
+	__syncthreads();
+	for (int o = 128; o > 0; o >>= 1) {
+		if (tid < o) {
+			float c2 = s_cc[tid + o];
+			int i2 = s_i[tid + o];
+			float c1 = s_cc[tid];
+			int i1 = s_i[tid];
+			if (i2 >= 0 && (i1 < 0 || c2 > c1 || (c2 == c1 && i2 < i1))) {
+				s_cc[tid] = c2; s_i[tid] = i2;
+			}
+		}
+		__syncthreads();
+	}
+	if (tid != 0) return;
+	SearchState s = s0;
+	const int win = s_i[0];
+	const float wcc = s_cc[0];
+	if (ran && win >= 0) {
+		float dfm, ast, ang, ph;
+		stage_candidate(s0, win, dfm, ast, ang, ph);
+		switch (s0.st_kind) {
+		case ST_INIT:
+			s.ccmax = wcc; s.oldcc = wcc;
+			break;
+		case ST_ASTIG:
+			if (wcc > s.ccmax) { s.ast = ast; s.ang = ang; s.ccmax = wcc; }
+			break;
+		case ST_ASTMAG:
+			if (wcc > s.ccmax) { s.ast = ast; s.ccmax = wcc; }
+			break;
+		case ST_ASTANG:
+			if (wcc > s.ccmax) { s.ang = ang; s.ccmax = wcc; }
+			break;
+		case ST_DFMEAN:
+			if (wcc > s.ccmax) { s.dfmean = dfm; s.ccmax = wcc; }
+			break;
+		case ST_PHASE:
+			if (wcc > s.ccmax) { s.phase = ph; s.ccmax = wcc; }
+			break;
+		default: break;
+		}
+	}
+	if (!s.done) {
+		switch (next) {
+		case NX_ASTIG:
+			prep_astig(s);
+			break;
+		case NX_ASTMAG:
+			if (s0.st_kind == ST_ASTIG || s.iter == 0) {
+				// after mFindAstig (CFindDefocus2D.cpp:158-166)
+				if (s.oldcc > s.ccmax) { s.done = 1; break; }
+				s.phase = s.phrange[0];
+				s.tot_df = s.dfrange[1] - s.dfrange[0];
+				s.tot_ast = s.astrange[1] - s.astrange[0];
+				s.tot_ang = s.angrange[1] - s.angrange[0];
+			}
+			s.iter += 1;
+			prep_line(s, ST_ASTMAG, s.ast, s.astrange, iter_scale(s.iter) * s.tot_ast, 21, 0.001f);
+			break;
+		case NX_ASTANG:
+			// CFindDefocus2D.cpp:171-173
+			if (s.oldcc > s.ccmax) { s.done = 1; break; }
+			s.ccmax = s.oldcc;
+			prep_line(s, ST_ASTANG, s.ang, s.angrange, iter_scale(s.iter) * s.tot_ang, 51, 1.0f);
+			break;
+		case NX_DFMEAN:
+			prep_dfmean(s, iter_scale(s.iter) * s.tot_df);
+			break;
+		case NX_PHASE:
+			prep_phase(s, iter_scale(s.iter) * s.phrange[1]);
+			break;
+		case NX_FINISH:
+			prep_none(s);
+			break;
+		default: break;
+		}
+	}
+	if (next == NX_FINISH || s.done) {
+		// CFindDefocus2D::GetDfMin/GetDfMax/... (CFindDefocus2D.cpp:38-67)
+		s.df_min = s.dfmean * (1.0f - s.ast);
+		s.df_max = s.dfmean * (1.0f + s.ast);
+		s.ast_ang = s.ang;
+		s.ext_phase = s.phase;
+		s.score = s.ccmax;
+		if (s.done) prep_none(s);
+	}
+	*S = s;
+}
+
+// ---------------------------------------------------------------------------
+// Spectral CC shells (GSpectralCC2D.cu:12-109) at the final parameters; the
+// CTF is evaluated in FP32 with the reference formula (GCalcCTF2D.cu:23-37).
+// Grid (cmpx, images).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_spectral_cc(CtfConsts cc, int cmpx, int cmpy, int width,
+	const float* __restrict__ spect, const SearchState* __restrict__ st, float* __restrict__ shell_cc)
+{
+	__shared__ double red[6][256];
+	const SearchState& S = st[blockIdx.y];
+	const int k = blockIdx.x;
+	int lo = k - width / 2, hi = lo + width;
+	if (lo < 0) { lo = 0; hi = width; }
+	else if (hi > cmpx) { lo = cmpx - width; hi = cmpx; }
+	float ph_rad = S.phase * kD2R2D, az = S.ang * kD2R2D;
+	float dmin = S.dfmean * (1.0f - S.ast) / cc.pix;
+	float dmax = S.dfmean * (1.0f + S.ast) / cc.pix;
+	float mean = 0.5f * (dmin + dmax), sig = 0.5f * (dmax - dmin);
+	float add = cc.amp_phase + ph_rad;
+	float w2 = cc.wl * cc.wl;
+	const float* sp = spect + (long long)blockIdx.y * cmpx * cmpy;
+	double a[6] = {0, 0, 0, 0, 0, 0};
+	const int ny = cmpy, nx = min(hi, cmpx);
+	for (int i = threadIdx.x; i < ny * nx; i += 256) {
+		int y = i / nx, x = i % nx;
+		float fy = fabsf((float)y - (float)cmpy * 0.5f);
+		if (fy >= (float)hi) continue;
+		float fr = sqrtf((float)(x * x) + fy * fy);
+		if (fr < (float)lo || fr >= (float)hi) continue;
+		float fX = (float)x * 0.5f / (float)(cmpx - 1);
+		float fY = (float)(y - cmpy / 2) / (float)cmpy;
+		float s2 = fX * fX + fY * fY;
+		float th = atanf(fY / (fX + (float)1e-30));
+		float df = mean + sig * cosf(2.0f * (th - az));
+		float c = -sinf(add + kPi2D * cc.wl * s2 * (df - 0.5f * s2 * w2 * cc.cs));
+		float s = sp[y * cmpx + x];
+		a[0] += c; a[1] += s; a[2] += (double)c * s; a[3] += (double)c * c; a[4] += (double)s * s; a[5] += 1.0;
+	}
+	for (int j = 0; j < 6; ++j) red[j][threadIdx.x] = a[j];
+	__syncthreads();
+	for (int o = 128; o > 0; o >>= 1) {
+		if (threadIdx.x < o)
+			for (int j = 0; j < 6; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + o];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		double n = red[5][0];
+		float out = 0.f;
+		if (n > 0) {
+			double mc = red[0][0] / n, ms = red[1][0] / n, mcs = red[2][0] / n;
+			double vc = red[3][0] / n - mc * mc, vs = red[4][0] / n - ms * ms;
+			if (vc > 0 && vs > 0) out = (float)((mcs - mc * ms) / sqrt(vc * vs));
+		}
+		shell_cc[(long long)blockIdx.y * cmpx + k] = out;
+	}
+}
+
+// GSpectralCC2D::DoIt shell pick (:157-171) and m_fCtfRes (CFindDefocus2D.cpp:399).
+__global__ void k_ctf_res(const float* __restrict__ shell_cc, int cmpx, int cmpy, float pix,
+	SearchState* __restrict__ st, int nimg)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= nimg) return;
+	const float* cc = shell_cc + (long long)b * cmpx;
+	int imax = 1;
+	float fmax = (float)-1e30;
+	for (int i = 1; i < cmpx; ++i) {
+		if (cc[i] <= fmax) continue;
+		fmax = cc[i]; imax = i;
+	}
+	int shell = imax;
+	if (!(fmax < 0.143f)) {
+		for (int i = imax; i < cmpx; ++i) {
+			if (cc[i] < 0.143f) break;
+			shell = i;
+		}
+	}
+	st[b].ctf_res = (float)cmpy * pix / (float)shell;
+}
+
+}  // namespace gctf

@@ -1,0 +1,486 @@
+// k_spectrum.hpp -- spectrum-stage kernels: Fourier-crop rescale, tiled
+// amplitude spectrum, 2D background removal, flatness gate, low-pass and
+// radial average.  Each kernel cites the reference function it replaces.
+#pragma once
+#include "gctf_internal.hpp"
+
+namespace gctf {
+
+__device__ __forceinline__ float load_px(const float* p, long long i) { return p[i]; }
+__device__ __forceinline__ float load_px(const _Float16* p, long long i) { return (float)p[i]; }
+
+// ---------------------------------------------------------------------------
+// Rescale (CRescaleImage.cpp:47-99; CCufft2D.cpp:23-114; GFtResize2D.cu:7-32)
+// ---------------------------------------------------------------------------
+
+// Real rows -> half-spectrum rows kept for kx < ncx_out.  Two rows share one
+// complex transform (z = a + i b).  Grid (ceil(ny/2), images).
+template <typename Tin, int NT, int MAXN>
+__global__ __launch_bounds__(NT) void k_rescale_rows_r2c(const Tin* __restrict__ img,
+	long long img_stride, int nx, int ny, Fft1dPlan plan, int ncx_out,
+	float2* __restrict__ out, long long out_stride)
+{
+	__shared__ float2 buf[MAXN];
+	const int tid = threadIdx.x;
+	const int y0 = 2 * blockIdx.x;
+	const bool two = (y0 + 1) < ny;
+	const Tin* src = img + blockIdx.y * img_stride + (long long)y0 * nx;
+	for (int x = tid; x < nx; x += NT) {
+		float a = load_px(src, x);
+		float b = two ? load_px(src, (long long)nx + x) : 0.f;
+		buf[x] = make_float2(a, b);
+	}
+	__syncthreads();
+	fft_any_lds<NT, MAXN, false>(buf, plan, tid);
+	float2* dst = out + blockIdx.y * out_stride + (long long)y0 * ncx_out;
+	for (int k = tid; k < ncx_out; k += NT) {
+		float2 z = buf[k];
+		float2 zc = buf[k == 0 ? 0 : nx - k];
+		// A = (Z[k] + conj Z[N-k]) / 2 ; B = -i (Z[k] - conj Z[N-k]) / 2
+		float2 A = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
+		float2 B = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+		dst[k] = A;
+		if (two) dst[ncx_out + k] = B;
+	}
+}
+
+// Complex transpose out[x][y] = in[y][x]; in is rows x cols.  32x32 tiles.
+__global__ __launch_bounds__(256) void k_transpose_c(const float2* __restrict__ in,
+	long long in_stride, int rows, int cols, float2* __restrict__ out, long long out_stride)
+{
+	__shared__ float2 t[32][33];
+	const float2* src = in + blockIdx.z * in_stride;
+	float2* dst = out + blockIdx.z * out_stride;
+	const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+	const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // ty in [0, 8)
+	for (int j = ty; j < 32; j += 8) {
+		int y = by + j, x = bx + tx;
+		if (y < rows && x < cols) t[j][tx] = src[(long long)y * cols + x];
+	}
+	__syncthreads();
+	for (int j = ty; j < 32; j += 8) {
+		int x = bx + j, y = by + tx;
+		if (x < cols && y < rows) dst[(long long)x * rows + y] = t[tx][j];
+	}
+}
+
+// Columns (rows of the transposed spectrum): forward length ny, crop to
+// nny rows (ky <= nny/2 from the top, the rest from the bottom), inverse
+// length nny.  Grid (ncx, images).
+template <int NT, int MAXN>
+__global__ __launch_bounds__(NT) void k_rescale_cols(const float2* __restrict__ in,
+	long long in_stride, int ny, int nny, Fft1dPlan pf, Fft1dPlan pi,
+	float2* __restrict__ out, long long out_stride)
+{
+	__shared__ float2 buf[MAXN];
+	const int tid = threadIdx.x;
+	const float2* src = in + blockIdx.y * in_stride + (long long)blockIdx.x * ny;
+	for (int y = tid; y < ny; y += NT) buf[y] = src[y];
+	__syncthreads();
+	fft_any_lds<NT, MAXN, false>(buf, pf, tid);
+	// crop: gather into registers, then rewrite the head of the buffer
+	constexpr int PER = (MAXN + NT - 1) / NT;
+	float2 v[PER];
+#pragma unroll
+	for (int i = 0; i < PER; ++i) {
+		int y = tid + i * NT;
+		if (y < nny) {
+			int ys = y > nny / 2 ? y - nny + ny : y;
+			v[i] = buf[ys];
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int i = 0; i < PER; ++i) {
+		int y = tid + i * NT;
+		if (y < nny) buf[y] = v[i];
+	}
+	__syncthreads();
+	fft_any_lds<NT, MAXN, true>(buf, pi, tid);
+	float2* dst = out + blockIdx.y * out_stride + (long long)blockIdx.x * nny;
+	for (int y = tid; y < nny; y += NT) dst[y] = buf[y];
+}
+
+// Half-spectrum rows -> real rows of length nnx (C2R; imaginary parts of the
+// DC and Nyquist bins are dropped, i.e. the real part of the inverse of the
+// Hermitian extension).  Two rows per transform.  Output rows have `pitch`
+// floats; the pad columns are zeroed.  Grid (ceil(nny/2), images).
+template <int NT, int MAXN>
+__global__ __launch_bounds__(NT) void k_rescale_rows_c2r(const float2* __restrict__ in,
+	long long in_stride, int nny, int nnx, Fft1dPlan plan, float* __restrict__ out,
+	long long out_stride, int pitch)
+{
+	__shared__ float2 buf[MAXN];
+	const int tid = threadIdx.x;
+	const int ncx = nnx / 2 + 1;
+	const int y0 = 2 * blockIdx.x;
+	const bool two = (y0 + 1) < nny;
+	const float2* ga = in + blockIdx.y * in_stride + (long long)y0 * ncx;
+	const float2* gb = ga + ncx;
+	for (int k = tid; k < nnx; k += NT) {
+		float2 a, b;
+		if (k < ncx) {
+			a = ga[k];
+			b = two ? gb[k] : make_float2(0.f, 0.f);
+			if (k == 0 || k == nnx / 2) { a.y = 0.f; b.y = 0.f; }
+		} else {
+			a = ga[nnx - k]; a.y = -a.y;
+			b = two ? gb[nnx - k] : make_float2(0.f, 0.f);
+			b.y = -b.y;
+		}
+		// Z = A + i B
+		buf[k] = make_float2(a.x - b.y, a.y + b.x);
+	}
+	__syncthreads();
+	fft_any_lds<NT, MAXN, true>(buf, plan, tid);
+	float* da = out + blockIdx.y * out_stride + (long long)y0 * pitch;
+	for (int x = tid; x < pitch; x += NT) {
+		float2 z = x < nnx ? buf[x] : make_float2(0.f, 0.f);
+		da[x] = z.x;
+		if (two) da[pitch + x] = z.y;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Tile spectrum (CGenAvgSpectrum.cpp:84-139, GCalcMoment2D.cu, GNormalize2D.cu,
+// GRoundEdge.cu, GCalcSpectrum.cu:15-31,90-105, GAddImages.cu)
+// ---------------------------------------------------------------------------
+
+struct TileGrid {
+	int ntx, nty, offx, offy, step, tile;
+	int blocks;          // 1: tile sums from 2x2 blocks of step x step
+	int nbx, nby;        // block counts
+	int bsize;           // block side (step, or tile when blocks == 0)
+};
+
+// FP64 sums of bsize x bsize blocks at (offx + i*step, offy + j*step).
+// Grid (nbx*nby, images), 256 threads.
+template <typename Tin>
+__global__ __launch_bounds__(256) void k_block_sums(const Tin* __restrict__ img,
+	long long img_stride, int pitch, TileGrid g, double* __restrict__ sums)
+{
+	__shared__ double red[256];
+	const int bi = blockIdx.x % g.nbx, bj = blockIdx.x / g.nbx;
+	const int sx = g.offx + bi * g.step, sy = g.offy + bj * g.step;
+	const Tin* src = img + blockIdx.y * img_stride;
+	const int tid = threadIdx.x;
+	double acc = 0.0;
+	for (int y = 0; y < g.bsize; ++y) {
+		const Tin* row = src + (long long)(sy + y) * pitch + sx;
+		float a = 0.f;
+		for (int x = tid; x < g.bsize; x += 256) a += load_px(row, x);
+		acc += (double)a;
+	}
+	red[tid] = acc;
+	__syncthreads();
+	for (int s = 128; s > 0; s >>= 1) {
+		if (tid < s) red[tid] += red[tid + s];
+		__syncthreads();
+	}
+	if (tid == 0) sums[(long long)blockIdx.y * g.nbx * g.nby + blockIdx.x] = red[0];
+}
+
+// Row pass of the tile FFT: for ROWS rows of one tile, subtract the tile
+// mean, apply the round-edge mask, R2C transform along x (two rows per
+// complex transform) and write the half-spectrum transposed:
+// rt[image][tile][kx][y].  Grid (ntiles * T/ROWS, images), (ROWS/2)*(T/8)
+// threads.
+template <typename Tin, int T, int ROWS>
+__global__ __launch_bounds__((ROWS / 2) * (T / 8)) void k_tile_rows(const Tin* __restrict__ img,
+	long long img_stride, int pitch, TileGrid g, const double* __restrict__ sums,
+	const float* __restrict__ mask, const float2* __restrict__ tw,
+	float2* __restrict__ rt, long long rt_stride)
+{
+	constexpr int FT = T / 8;              // threads per transform
+	constexpr int NG = ROWS / 2;           // transforms per workgroup
+	constexpr int NT = NG * FT;
+	constexpr int CX = T / 2 + 1;
+	__shared__ float2 buf[NG * T];
+	__shared__ float2 stage[CX * ROWS];
+	const int rb = blockIdx.x % (T / ROWS);
+	const int t = blockIdx.x / (T / ROWS);
+	const int tx = t % g.ntx, ty = t / g.ntx;
+	const int sx = g.offx + tx * g.step, sy = g.offy + ty * g.step;
+	const long long nb = (long long)g.nbx * g.nby;
+	const double* bs = sums + blockIdx.y * nb;
+	double tsum;
+	if (g.blocks) {
+		tsum = bs[ty * g.nbx + tx] + bs[ty * g.nbx + tx + 1] + bs[(ty + 1) * g.nbx + tx]
+			+ bs[(ty + 1) * g.nbx + tx + 1];
+	} else {
+		tsum = bs[ty * g.nbx + tx];
+	}
+	const float mean = (float)(tsum / ((double)T * (double)T));
+	const int grp = threadIdx.x / FT, lane = threadIdx.x % FT;
+	const int ya = rb * ROWS + 2 * grp;
+	const Tin* src = img + blockIdx.y * img_stride + (long long)(sy + ya) * pitch + sx;
+	float2* fb = buf + grp * T;
+	for (int x = lane; x < T; x += FT) {
+		float a = (load_px(src, x) - mean) * mask[ya * T + x];
+		float b = (load_px(src, (long long)pitch + x) - mean) * mask[(ya + 1) * T + x];
+		fb[x] = make_float2(a, b);
+	}
+	__syncthreads();
+	fft_lds<T, FT, false>(fb, tw, lane);
+	for (int k = lane; k < CX; k += FT) {
+		float2 z = fb[k];
+		float2 zc = fb[(T - k) & (T - 1)];
+		float2 A = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
+		float2 B = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+		stage[k * ROWS + 2 * grp] = A;
+		stage[k * ROWS + 2 * grp + 1] = B;
+	}
+	__syncthreads();
+	float2* dst = rt + blockIdx.y * rt_stride + (long long)t * CX * T + rb * ROWS;
+	for (int i = threadIdx.x; i < CX * ROWS; i += NT) {
+		int k = i / ROWS, yy = i % ROWS;
+		dst[(long long)k * T + yy] = stage[i];
+	}
+}
+
+// Column pass: for CPW kx-columns and the tiles [t0, t1) of split s,
+// transform along y, take |F| and accumulate tile * (1/ntiles).  Writes the
+// DC-shifted partial average part[image][s][y][kx].  Grid
+// (ceil(CX/CPW), nsplit, images), CPW*(T/8) threads.
+template <int T, int CPW>
+__global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols(const float2* __restrict__ rt,
+	long long rt_stride, int ntiles, int nsplit, float fac, const float2* __restrict__ tw,
+	float* __restrict__ part)
+{
+	constexpr int FT = T / 8;
+	constexpr int CX = T / 2 + 1;
+	constexpr int PER = T / FT;            // 8 outputs per thread
+	__shared__ float2 buf[CPW * T];
+	const int grp = threadIdx.x / FT, lane = threadIdx.x % FT;
+	const int kx = blockIdx.x * CPW + grp;
+	const bool live = kx < CX;
+	const int s = blockIdx.y;
+	const int per = (ntiles + nsplit - 1) / nsplit;
+	const int t0 = s * per, t1 = min(ntiles, t0 + per);
+	float acc[PER];
+#pragma unroll
+	for (int i = 0; i < PER; ++i) acc[i] = 0.f;
+	float2* fb = buf + grp * T;
+	const float2* src = rt + blockIdx.z * rt_stride;
+	for (int t = t0; t < t1; ++t) {
+		if (live) {
+			const float2* col = src + ((long long)t * CX + kx) * T;
+			for (int y = lane; y < T; y += FT) fb[y] = col[y];
+		}
+		__syncthreads();
+		fft_lds<T, FT, false>(fb, tw, lane);
+#pragma unroll
+		for (int i = 0; i < PER; ++i) {
+			float2 c = fb[lane + i * FT];
+			acc[i] = fmaf(sqrtf(fmaf(c.x, c.x, c.y * c.y)), fac, acc[i]);
+		}
+		__syncthreads();
+	}
+	if (!live) return;
+	float* dst = part + ((long long)blockIdx.z * nsplit + s) * (long long)T * CX;
+#pragma unroll
+	for (int i = 0; i < PER; ++i) {
+		int y = lane + i * FT;
+		int yd = (y + T / 2) & (T - 1);
+		dst[(long long)yd * CX + kx] = acc[i];
+	}
+}
+
+// Sum the split partials in split order: raw[image][i] = sum_s part[s][i].
+__global__ __launch_bounds__(256) void k_tile_reduce(const float* __restrict__ part, int nsplit,
+	int n, float* __restrict__ raw)
+{
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= n) return;
+	const float* p = part + (long long)blockIdx.y * nsplit * n;
+	float acc = 0.f;
+	for (int s = 0; s < nsplit; ++s) acc += p[(long long)s * n + i];
+	raw[(long long)blockIdx.y * n + i] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Background (GRmBackground2D.cu:11-94): out = in - (box mean of sqrt(in))^2
+// over a periodic, Friedel-unfolded plane P(X, Y), X, Y in [0, T):
+//   X <  T/2: P = in[Y][X];  X >= T/2: P = in[(T-Y) mod T][T-X].
+// Box sums are separable: H = row sums, then column sums of H.
+// ---------------------------------------------------------------------------
+template <int T>
+__global__ __launch_bounds__(256) void k_bg_rows(const float* __restrict__ raw, int log_spect,
+	int half_box, float* __restrict__ hsum)
+{
+	constexpr int CX = T / 2 + 1;
+	__shared__ float p[T];
+	const float* in = raw + (long long)blockIdx.y * T * CX;
+	float* out = hsum + (long long)blockIdx.y * T * CX;
+	const int Y = blockIdx.x;
+	for (int X = threadIdx.x; X < T; X += 256) {
+		float v = X < T / 2 ? in[Y * CX + X] : in[((T - Y) & (T - 1)) * CX + (T - X)];
+		p[X] = log_spect ? v : sqrtf(v);
+	}
+	__syncthreads();
+	for (int x = threadIdx.x; x < CX; x += 256) {
+		float acc = 0.f;
+		for (int j = -half_box; j <= half_box; ++j) acc += p[(x + j + T) & (T - 1)];
+		out[Y * CX + x] = acc;
+	}
+}
+
+// Column sums, the subtraction, the r < 0.04 zeroing (evaluated in float as
+// the reference does) and per-workgroup moment partials for the gate.
+// Grid (ceil(T*CX/256), images).
+template <int T>
+__global__ __launch_bounds__(256) void k_bg_cols(const float* __restrict__ raw,
+	const float* __restrict__ hsum, int log_spect, int half_box, float* __restrict__ spect,
+	double* __restrict__ mom)
+{
+	constexpr int CX = T / 2 + 1;
+	__shared__ double r1[256], r2[256];
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	const float* in = raw + (long long)blockIdx.z * T * CX;
+	const float* h = hsum + (long long)blockIdx.z * T * CX;
+	double m1 = 0.0, m2 = 0.0;
+	if (i < T * CX) {
+		const int y = i / CX, x = i % CX;
+		const int yc = y - T / 2;
+		float fr = (float)x * 0.5f / (float)(CX - 1);
+		fr = sqrtf(fr * fr + (float)(yc * yc) / (float)(T * T));
+		float o;
+		if ((double)fr < 0.04) {
+			o = 0.f;
+		} else {
+			float acc = 0.f;
+			for (int k = -half_box; k <= half_box; ++k) acc += h[((y + k + T) & (T - 1)) * CX + x];
+			const int b = 2 * half_box + 1;
+			acc /= (float)(b * b);
+			if (!log_spect) acc *= acc;
+			o = in[i] - acc;
+		}
+		spect[(long long)blockIdx.z * T * CX + i] = o;
+		m1 = o; m2 = (double)o * o;
+	}
+	r1[threadIdx.x] = m1; r2[threadIdx.x] = m2;
+	__syncthreads();
+	for (int s = 128; s > 0; s >>= 1) {
+		if (threadIdx.x < s) { r1[threadIdx.x] += r1[threadIdx.x + s]; r2[threadIdx.x] += r2[threadIdx.x + s]; }
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		double* m = mom + ((long long)blockIdx.z * gridDim.x + blockIdx.x) * 2;
+		m[0] = r1[0]; m[1] = r2[0];
+	}
+}
+
+// Flatness gate (CFindCtfBase.cpp:167-174): low-pass iff var >= 1.
+__global__ __launch_bounds__(256) void k_gate(const double* __restrict__ mom, int nparts, int n,
+	SearchState* __restrict__ st)
+{
+	__shared__ double r1[256], r2[256];
+	const double* m = mom + (long long)blockIdx.x * nparts * 2;
+	double a = 0.0, b = 0.0;
+	for (int i = threadIdx.x; i < nparts; i += 256) { a += m[2 * i]; b += m[2 * i + 1]; }
+	r1[threadIdx.x] = a; r2[threadIdx.x] = b;
+	__syncthreads();
+	for (int s = 128; s > 0; s >>= 1) {
+		if (threadIdx.x < s) { r1[threadIdx.x] += r1[threadIdx.x + s]; r2[threadIdx.x] += r2[threadIdx.x + s]; }
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		double mean = r1[0] / n, m2 = r2[0] / n;
+		st[blockIdx.x].lowpassed = (m2 - mean * mean) < 1.0 ? 0 : 1;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Low-pass (CFindCtfBase.cpp:191-219, GLowpass2D.cu:9-23): the transfer
+// exp(s (kx^2 + ky^2)) is separable, so the 2D FFT round trip equals a
+// circular 1D round trip along x and then along y, each normalised by 1/T.
+// x pass: rows of the unfolded full plane (GCalcSpectrum.cu:47-69, row T
+// taken mod T); only the columns T/2 .. T (mod T) are kept.
+// ---------------------------------------------------------------------------
+template <int T>
+__global__ __launch_bounds__(2 * (T / 8)) void k_lp_x(const float* __restrict__ spect,
+	const SearchState* __restrict__ st, const float* __restrict__ h1,
+	const float2* __restrict__ tw, float* __restrict__ q)
+{
+	constexpr int CX = T / 2 + 1, FT = T / 8, HX = T / 2;
+	__shared__ float2 buf[2 * T];
+	if (!st[blockIdx.y].lowpassed) return;
+	const int grp = threadIdx.x / FT, lane = threadIdx.x % FT;
+	const int ya = 4 * blockIdx.x + 2 * grp;     // rows ya, ya + 1
+	const float* in = spect + (long long)blockIdx.y * T * CX;
+	float2* fb = buf + grp * T;
+	for (int xd = lane; xd < T; xd += FT) {
+		float v[2];
+#pragma unroll
+		for (int r = 0; r < 2; ++r) {
+			int yd = ya + r, xs = xd - HX, ys = yd;
+			if (xs < 0) { xs = -xs; ys = (T - yd) & (T - 1); }
+			v[r] = in[ys * CX + xs];
+		}
+		fb[xd] = make_float2(v[0], v[1]);
+	}
+	__syncthreads();
+	fft_lds<T, FT, false>(fb, tw, lane);
+	for (int k = lane; k < T; k += FT) {
+		float f = h1[k];
+		fb[k] = make_float2(fb[k].x * f, fb[k].y * f);
+	}
+	__syncthreads();
+	fft_lds<T, FT, true>(fb, tw, lane);
+	float* out = q + (long long)blockIdx.y * T * CX;
+	const float inv = 1.0f / (float)T;
+	for (int c = lane; c < CX; c += FT) {
+		float2 z = fb[(HX + c) & (T - 1)];
+		out[ya * CX + c] = z.x * inv;
+		out[(ya + 1) * CX + c] = z.y * inv;
+	}
+}
+
+// y pass on columns c, c+1 of q; result into spect.  Grid (ceil(CX/2), images).
+template <int T>
+__global__ __launch_bounds__(T / 8) void k_lp_y(const float* __restrict__ q,
+	const SearchState* __restrict__ st, const float* __restrict__ h1,
+	const float2* __restrict__ tw, float* __restrict__ spect)
+{
+	constexpr int CX = T / 2 + 1, FT = T / 8;
+	__shared__ float2 buf[T];
+	if (!st[blockIdx.y].lowpassed) return;
+	const int c0 = 2 * blockIdx.x;
+	const bool two = c0 + 1 < CX;
+	const float* in = q + (long long)blockIdx.y * T * CX;
+	const int lane = threadIdx.x;
+	for (int y = lane; y < T; y += FT)
+		buf[y] = make_float2(in[y * CX + c0], two ? in[y * CX + c0 + 1] : 0.f);
+	__syncthreads();
+	fft_lds<T, FT, false>(buf, tw, lane);
+	for (int k = lane; k < T; k += FT) {
+		float f = h1[k];
+		buf[k] = make_float2(buf[k].x * f, buf[k].y * f);
+	}
+	__syncthreads();
+	fft_lds<T, FT, true>(buf, tw, lane);
+	float* out = spect + (long long)blockIdx.y * T * CX;
+	const float inv = 1.0f / (float)T;
+	for (int y = lane; y < T; y += FT) {
+		out[y * CX + c0] = buf[y].x * inv;
+		if (two) out[y * CX + c0 + 1] = buf[y].y * inv;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Radial average (GRadialAvg.cu:11-33) from a host-built FP32 sample table.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_radial(const float* __restrict__ spect, int cmpx, int cmpy,
+	const int* __restrict__ roff, const int* __restrict__ rcnt, const int* __restrict__ ridx,
+	float* __restrict__ radial)
+{
+	const float* in = spect + (long long)blockIdx.x * cmpx * cmpy;
+	for (int r = threadIdx.x; r < cmpx; r += 256) {
+		double s = 0.0;
+		const int o = roff[r], n = rcnt[r];
+		for (int i = 0; i < n; ++i) s += (double)in[ridx[o + i]];
+		radial[(long long)blockIdx.x * cmpx + r] = (float)(s / n);
+	}
+}
+
+}  // namespace gctf

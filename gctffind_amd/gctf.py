@@ -1,0 +1,267 @@
+"""ctypes binding of libgctf_hip.so (include/gctf_hip.h).
+
+The classes mirror the reference's C++ call surface (FindCTF/CFindCTFInc.h):
+CRescaleImage, CCTFTheory and CFindCtf2D, with the same method names and
+argument meaning, so the parity tests read like calls into the reference.
+Device buffers are passed as raw addresses; torch is only used by callers
+to own device memory.
+
+The library must be built (python -m gctffind_amd.build); loading fails
+loudly when it is missing.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgctf_hip.so")
+
+GCTF_OK, GCTF_ERR_ARG, GCTF_ERR_HIP, GCTF_ERR_STATE, GCTF_ERR_UNSUPPORTED, GCTF_ERR_NOMEM = range(6)
+DTYPE_F32, DTYPE_F16 = 0, 1
+
+EXPORTS = [
+    "gctf_create", "gctf_destroy", "gctf_error_string", "gctf_set_stream", "gctf_synchronize",
+    "gctf_rescale_setup", "gctf_rescale", "gctf_scaled_image", "gctf_setup1", "gctf_setup2",
+    "gctf_set_phase", "gctf_gen_half_spectrum", "gctf_do1d", "gctf_refine1d", "gctf_do2d",
+    "gctf_refine", "gctf_spect_size", "gctf_get_half_spect", "gctf_set_half_spect",
+    "gctf_full_spectrum", "gctf_estimate_batch", "gctf_tune",
+]
+
+
+class GctfParams(ctypes.Structure):
+    _fields_ = [("kv", ctypes.c_float), ("cs_mm", ctypes.c_float),
+                ("amp_contrast", ctypes.c_float), ("ext_phase_deg", ctypes.c_float),
+                ("ext_phase_range_deg", ctypes.c_float), ("tile_size", ctypes.c_int),
+                ("log_spect", ctypes.c_int)]
+
+
+class GctfResult(ctypes.Structure):
+    _fields_ = [("df_min", ctypes.c_float), ("df_max", ctypes.c_float),
+                ("azimuth_deg", ctypes.c_float), ("ext_phase_deg", ctypes.c_float),
+                ("score", ctypes.c_float), ("ctf_res", ctypes.c_float)]
+
+    def as_dict(self):
+        return dict(df_min=self.df_min, df_max=self.df_max, azimuth=self.azimuth_deg,
+                    ext_phase=self.ext_phase_deg, score=self.score, ctf_res=self.ctf_res)
+
+
+class GctfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gctf error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libgctf_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m gctffind_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ci, cf, fp = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_void_p
+    sig = {
+        "gctf_create": (ci, [ctypes.POINTER(GctfParams), ci, ctypes.POINTER(vp)]),
+        "gctf_destroy": (None, [vp]),
+        "gctf_error_string": (ctypes.c_char_p, [vp]),
+        "gctf_set_stream": (ci, [vp, vp]),
+        "gctf_synchronize": (ci, [vp]),
+        "gctf_rescale_setup": (ci, [vp, ci, ci, cf, fp, fp]),
+        "gctf_rescale": (ci, [vp, fp, ci]),
+        "gctf_scaled_image": (ci, [vp, ctypes.POINTER(vp)]),
+        "gctf_setup1": (ci, [vp, cf, cf, cf, cf, cf, cf]),
+        "gctf_setup2": (ci, [vp, ci, ci]),
+        "gctf_set_phase": (ci, [vp, cf, cf]),
+        "gctf_gen_half_spectrum": (ci, [vp, fp]),
+        "gctf_do1d": (ci, [vp, ctypes.POINTER(GctfResult)]),
+        "gctf_refine1d": (ci, [vp, cf, cf, ctypes.POINTER(GctfResult)]),
+        "gctf_do2d": (ci, [vp, ctypes.POINTER(GctfResult)]),
+        "gctf_refine": (ci, [vp, fp, fp, fp, fp, ctypes.POINTER(GctfResult)]),
+        "gctf_spect_size": (ci, [vp, fp, ci]),
+        "gctf_get_half_spect": (ci, [vp, ci, fp]),
+        "gctf_set_half_spect": (ci, [vp, fp]),
+        "gctf_full_spectrum": (ci, [vp, fp]),
+        "gctf_estimate_batch": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, fp]),
+        "gctf_tune": (ci, [vp, ci, ci, ci]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One gctf_ctx: a GPU, a stream and the CFindCtf2D-shaped state."""
+
+    def __init__(self, kv=300.0, cs=2.7, ac=0.07, ext_phase=(0.0, 0.0), tile=512, log_spect=0,
+                 device=0):
+        self.L = load()
+        self.params = GctfParams(kv, cs, ac, ext_phase[0], ext_phase[1], tile, log_spect)
+        h = ctypes.c_void_p()
+        rc = self.L.gctf_create(ctypes.byref(self.params), device, ctypes.byref(h))
+        if rc != GCTF_OK:
+            raise GctfError(rc, "gctf_create failed")
+        self.h = h
+        self.tile = tile
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gctf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != GCTF_OK:
+            msg = self.L.gctf_error_string(self.h)
+            raise GctfError(rc, msg.decode() if msg else "")
+        return rc
+
+    # ---- CRescaleImage ----------------------------------------------------
+    def rescale_setup(self, nx, ny, pix):
+        size = np.zeros(3, np.int32)
+        pixn = np.zeros(1, np.float32)
+        self.check(self.L.gctf_rescale_setup(self.h, nx, ny, pix, _np_ptr(size), _np_ptr(pixn)))
+        return int(size[0]), int(size[1]), int(size[2]), float(pixn[0])
+
+    def rescale(self, img, on_device=False):
+        if on_device:
+            self.check(self.L.gctf_rescale(self.h, ctypes.c_void_p(img), 1))
+        else:
+            img = np.ascontiguousarray(img, np.float32)
+            self.check(self.L.gctf_rescale(self.h, _np_ptr(img), 0))
+
+    def scaled_image(self):
+        p = ctypes.c_void_p()
+        self.check(self.L.gctf_scaled_image(self.h, ctypes.byref(p)))
+        return p.value
+
+    # ---- CCTFTheory + CFindCtf2D ----------------------------------------
+    def setup1(self, kv, cs, ac, pix, ast_tol=100.0, ext_phase_rad=0.0):
+        self.check(self.L.gctf_setup1(self.h, kv, cs, ac, pix, ast_tol, ext_phase_rad))
+
+    def setup2(self, nx, ny):
+        self.check(self.L.gctf_setup2(self.h, nx, ny))
+
+    def set_phase(self, p0, rng):
+        self.check(self.L.gctf_set_phase(self.h, p0, rng))
+
+    def gen_half_spectrum(self, dev_pad_img):
+        self.check(self.L.gctf_gen_half_spectrum(self.h, ctypes.c_void_p(dev_pad_img)))
+
+    def do1d(self):
+        r = GctfResult()
+        self.check(self.L.gctf_do1d(self.h, ctypes.byref(r)))
+        return r.as_dict()
+
+    def refine1d(self, df, rng):
+        r = GctfResult()
+        self.check(self.L.gctf_refine1d(self.h, df, rng, ctypes.byref(r)))
+        return r.as_dict()
+
+    def do2d(self):
+        r = GctfResult()
+        self.check(self.L.gctf_do2d(self.h, ctypes.byref(r)))
+        return r.as_dict()
+
+    def refine(self, df_mean, ast_ratio, ast_angle, ext_phase):
+        a = [np.asarray(v, np.float32) for v in (df_mean, ast_ratio, ast_angle, ext_phase)]
+        r = GctfResult()
+        self.check(self.L.gctf_refine(self.h, *[_np_ptr(v) for v in a], ctypes.byref(r)))
+        return r.as_dict()
+
+    def spect_size(self, half=True):
+        s = np.zeros(2, np.int32)
+        self.check(self.L.gctf_spect_size(self.h, _np_ptr(s), int(half)))
+        return int(s[0]), int(s[1])
+
+    def get_half_spect(self, raw=False):
+        out = np.zeros((self.tile, self.tile // 2 + 1), np.float32)
+        self.check(self.L.gctf_get_half_spect(self.h, int(raw), _np_ptr(out)))
+        return out
+
+    def set_half_spect(self, spect):
+        spect = np.ascontiguousarray(spect, np.float32)
+        self.check(self.L.gctf_set_half_spect(self.h, _np_ptr(spect)))
+
+    def full_spectrum(self):
+        out = np.zeros((self.tile, self.tile), np.float32)
+        self.check(self.L.gctf_full_spectrum(self.h, _np_ptr(out)))
+        return out
+
+    # ---- batched path ----------------------------------------------------
+    def estimate_batch(self, dev_ptr, n, nx, ny, pix, dtype=DTYPE_F32, stride=None):
+        stride = nx * ny if stride is None else stride
+        res = (GctfResult * n)()
+        self.check(self.L.gctf_estimate_batch(self.h, ctypes.c_void_p(dev_ptr), dtype, n, nx, ny,
+                                              stride, pix, ctypes.cast(res, ctypes.c_void_p)))
+        return [r.as_dict() for r in res]
+
+    def tune(self, chunk=0, tile_splits=0, use_graph=0):
+        self.check(self.L.gctf_tune(self.h, chunk, tile_splits, use_graph))
+
+    def synchronize(self):
+        self.check(self.L.gctf_synchronize(self.h))
+
+
+class CFindCtf2D:
+    """CFindCtf2D-shaped facade (CFindCTFInc.h:541-615) over a Context."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.m_fDfMin = self.m_fDfMax = self.m_fAstAng = self.m_fExtPhase = self.m_fScore = 0.0
+
+    def _take(self, r):
+        self.m_fDfMin, self.m_fDfMax = r["df_min"], r["df_max"]
+        self.m_fAstAng, self.m_fExtPhase, self.m_fScore = r["azimuth"], r["ext_phase"], r["score"]
+        return r
+
+    def Setup1(self, kv, cs, ac, pix, ast_tol=100.0, ext_phase_rad=0.0):
+        self.ctx.setup1(kv, cs, ac, pix, ast_tol, ext_phase_rad)
+
+    def Setup2(self, size):
+        self.ctx.setup2(int(size[0]), int(size[1]))
+
+    def SetPhase(self, p0, rng):
+        self.ctx.set_phase(p0, rng)
+
+    def GenHalfSpectrum(self, dev_pad_img):
+        self.ctx.gen_half_spectrum(dev_pad_img)
+
+    def Do1D(self):
+        return self._take(self.ctx.do1d())
+
+    def Refine1D(self, df, rng):
+        return self._take(self.ctx.refine1d(df, rng))
+
+    def Do2D(self):
+        return self._take(self.ctx.do2d())
+
+    def Refine(self, df_mean, ast_ratio, ast_angle, ext_phase):
+        return self._take(self.ctx.refine(df_mean, ast_ratio, ast_angle, ext_phase))
+
+    def GetHalfSpect(self, raw=False):
+        return self.ctx.get_half_spect(raw)
+
+    def SetHalfSpect(self, spect):
+        self.ctx.set_half_spect(spect)
+
+    def GetSpectSize(self, half=True):
+        return self.ctx.spect_size(half)
+
+    def GenFullSpectrum(self):
+        return self.ctx.full_spectrum()

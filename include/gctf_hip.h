@@ -1,0 +1,129 @@
+/*
+ * gctf_hip.h -- C-ABI of libgctf_hip.so, the MI355X (gfx950) CTF-estimation
+ * path of GCtfFind.
+ *
+ * Plain C types only: pointers, sizes, floats.  Every entry point returns a
+ * GCTF_* status; gctf_error_string() describes the last failure of a
+ * context.  One context drives one GPU; calls on one context must not
+ * overlap (the context owns its device buffers and stream).
+ *
+ * Each declaration names the reference interface it replaces
+ * (FindCTF/CFindCTFInc.h in the reference tree, file:line).  The
+ * reference has no C-ABI; its call surface is the C++ classes
+ * CRescaleImage, CCTFTheory, CFindCtfBase, CFindCtf1D and CFindCtf2D.
+ * gctffind_amd/csrc/host/ re-creates those classes on top of this ABI.
+ */
+#ifndef GCTF_HIP_H
+#define GCTF_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GCTF_API __attribute__((visibility("default")))
+
+#define GCTF_OK 0
+#define GCTF_ERR_ARG 1          /* invalid argument (null pointer, size, range) */
+#define GCTF_ERR_HIP 2          /* a HIP runtime call failed */
+#define GCTF_ERR_STATE 3        /* call order violated (e.g. Do2D before GenHalfSpectrum) */
+#define GCTF_ERR_UNSUPPORTED 4  /* shape/tile outside what the kernels handle */
+#define GCTF_ERR_NOMEM 5        /* device allocation failed */
+
+#define GCTF_DTYPE_F32 0
+#define GCTF_DTYPE_F16 1
+
+typedef struct gctf_ctx gctf_ctx;
+
+/* Command-line parameters of CInput (CInput.cpp:42-53, 113-179). */
+typedef struct {
+	float kv;                  /* -kV          (300)  */
+	float cs_mm;               /* -Cs          (2.7)  */
+	float amp_contrast;        /* -AmpContrast (0.07) */
+	float ext_phase_deg;       /* -ExtPhase p0 (0)    */
+	float ext_phase_range_deg; /* -ExtPhase range (0: no phase search) */
+	int tile_size;             /* -TileSize    (512)  */
+	int log_spect;             /* -LogSpect    (0)    */
+} gctf_params;
+
+/* CFindCtfBase public results (CFindCTFInc.h:558-562) plus
+ * CFindDefocus2D::GetCtfRes (CFindCTFInc.h:499). */
+typedef struct {
+	float df_min;        /* A */
+	float df_max;        /* A */
+	float azimuth_deg;   /* astigmatism azimuth, dfMax axis from +x towards +y */
+	float ext_phase_deg;
+	float score;         /* CC */
+	float ctf_res;       /* A (GSpectralCC2D shell) */
+} gctf_result;
+
+/* Creates a context on HIP device `device`; *out is null on failure. */
+GCTF_API int gctf_create(const gctf_params* params, int device, gctf_ctx** out);
+GCTF_API void gctf_destroy(gctf_ctx* ctx);
+GCTF_API const char* gctf_error_string(const gctf_ctx* ctx);
+/* Launch on a caller-owned hipStream_t (null: the context's own stream). */
+GCTF_API int gctf_set_stream(gctf_ctx* ctx, void* hip_stream);
+/* Wait for all work of the context. */
+GCTF_API int gctf_synchronize(gctf_ctx* ctx);
+
+/* ---- CRescaleImage (CFindCTFInc.h:422-442) ---------------------------- */
+/* CRescaleImage::Setup(int* piRawSize, float fRawPixSize) (:428).
+ * new_size[0..2] = new nx, new ny, padded pitch (floats); *new_pix =
+ * m_fPixSizeN. */
+GCTF_API int gctf_rescale_setup(gctf_ctx* ctx, int nx, int ny, float raw_pix, int* new_size,
+	float* new_pix);
+/* CRescaleImage::DoIt(float* pfImage) (:429): nx*ny float32 image, host
+ * memory when img_on_device == 0, else device memory. */
+GCTF_API int gctf_rescale(gctf_ctx* ctx, const float* img, int img_on_device);
+/* CRescaleImage::GetScaledImg() (:430): device pointer, padded rows. */
+GCTF_API int gctf_scaled_image(gctf_ctx* ctx, float** dev_pad_img);
+
+/* ---- CCTFTheory + CFindCtf2D (CFindCTFInc.h:31-91, 541-615) ------------ */
+/* CCTFTheory::Setup(kV, Cs, AmpContrast, PixelSize, AstTol, ExtPhase)
+ * (:36-43) followed by CFindCtf2D::Setup1(CCTFTheory*) (:604). */
+GCTF_API int gctf_setup1(gctf_ctx* ctx, float kv, float cs_mm, float amp_contrast, float pix,
+	float ast_tol, float ext_phase_rad);
+/* CFindCtfBase::Setup2(int* piImgSize) (:548). */
+GCTF_API int gctf_setup2(gctf_ctx* ctx, int nx, int ny);
+/* CFindCtfBase::SetPhase(float fInitPhase, float fPhaseRange) (:549). */
+GCTF_API int gctf_set_phase(gctf_ctx* ctx, float p0_deg, float range_deg);
+/* CFindCtfBase::GenHalfSpectrum(float* gfPadImg) (:553): device image of
+ * Setup2's size with row pitch (nx/2+1)*2 floats. */
+GCTF_API int gctf_gen_half_spectrum(gctf_ctx* ctx, const float* dev_pad_img);
+/* CFindCtf1D::Do1D() / Refine1D(float, float) (:588-589). */
+GCTF_API int gctf_do1d(gctf_ctx* ctx, gctf_result* out);
+GCTF_API int gctf_refine1d(gctf_ctx* ctx, float init_df, float df_range, gctf_result* out);
+/* CFindCtf2D::Do2D() (:605). */
+GCTF_API int gctf_do2d(gctf_ctx* ctx, gctf_result* out);
+/* CFindCtf2D::Refine(afDfMean[2], afAstRatio[2], afAstAngle[2],
+ * afExtPhase[2]) (:606-611); [0] = centre, [1] = full range. */
+GCTF_API int gctf_refine(gctf_ctx* ctx, const float* df_mean, const float* ast_ratio,
+	const float* ast_angle_deg, const float* ext_phase_deg, gctf_result* out);
+/* CFindCtfBase::GetSpectSize(int*, bool bHalf) (:552). */
+GCTF_API int gctf_spect_size(gctf_ctx* ctx, int* size, int half);
+/* CFindCtfBase::GetHalfSpect(bool bRaw, bool bToHost=true) (:551): copies
+ * (tile/2+1) x tile floats into host_out. */
+GCTF_API int gctf_get_half_spect(gctf_ctx* ctx, int raw, float* host_out);
+/* CFindCtfBase::SetHalfSpect(float*) (:550): host spectrum replaces the
+ * background-removed spectrum. */
+GCTF_API int gctf_set_half_spect(gctf_ctx* ctx, const float* host_spect);
+/* CFindCtfBase::GenFullSpectrum() (:554): tile x tile floats, the fitted
+ * |CTF| painted on the left half within the resolution ring. */
+GCTF_API int gctf_full_spectrum(gctf_ctx* ctx, float* host_out);
+
+/* ---- Batched path (no reference counterpart) --------------------------
+ * Rescale + GenHalfSpectrum + Do2D (the reference's full-search package,
+ * CFindSeriesCtfs.cpp:108-141) for n micrographs of one shape and pixel
+ * size, device-resident, image i at dev_imgs + i * stride elements of
+ * `dtype`.  The whole pipeline of a batch is captured once into a HIP graph
+ * and replayed.  out: n results (host memory). */
+GCTF_API int gctf_estimate_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx,
+	int ny, long long stride, float pix, gctf_result* out);
+
+/* Knobs of the batched path (0 keeps the current value): images per
+ * spectrum chunk, tile-column splits, graph use (1 on, -1 off). */
+GCTF_API int gctf_tune(gctf_ctx* ctx, int chunk, int tile_splits, int use_graph);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GCTF_HIP_H */
