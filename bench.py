@@ -103,7 +103,90 @@ def main():
     }
     dfe = [abs(0.5 * (r["df_min"] + r["df_max"]) - t[0]) for r, t in zip(res, truth)]
     out["config"]["median_abs_ddf_vs_truth_A"] = round(float(sorted(dfe)[len(dfe) // 2]), 1)
+
+    if not a.no_roofline:
+        out["roofline"], out["kernels"] = roofline(ctx, ptr, n, a)
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"], par = cpu_baseline(imgs[0].cpu().numpy(), a.pix, res[0])
+        out["config"]["parity_vs_oracle_img0"] = par
     print(json.dumps(out), flush=True)
+
+
+def geometry(size, pix, tile=512):
+    """Shapes of the plain path for a size x size image (CRescaleImage /
+    CGenAvgSpectrum rules, as the oracle computes them)."""
+    import oracle as O_  # shape rules only (integer/float32 arithmetic)
+    nnx, nny, pitch, pixn, b = O_.rescale_sizes(size, size, pix)
+    ntx, nty, ox, oy, ov, step = O_.tile_grid(nnx, nny, tile)
+    return dict(nx=size, ny=size, nnx=nnx, nny=nny, pitch=pitch, ncx=nnx // 2 + 1, ntiles=ntx * nty,
+                cover=((ntx + 1) * step) * ((nty + 1) * step), cx=tile // 2 + 1, T=tile)
+
+
+def kernel_bytes(name, g, per_launch_images, nsplit=8):
+    """Algorithmic HBM bytes of one launch: the compulsory reads of its input
+    and writes of its output for the images it processes."""
+    c = per_launch_images
+    if name == "k_rescale_rows_r2c":
+        b = g["ny"] * g["nx"] * 4 + g["ny"] * g["ncx"] * 8
+    elif name == "k_transpose_c":
+        b = 0.5 * (2 * g["ny"] * g["ncx"] * 8 + 2 * g["ncx"] * g["nny"] * 8)
+    elif name == "k_rescale_cols":
+        b = g["ncx"] * g["ny"] * 8 + g["ncx"] * g["nny"] * 8
+    elif name == "k_rescale_rows_c2r":
+        b = g["nny"] * g["ncx"] * 8 + g["nny"] * g["pitch"] * 4
+    elif name == "k_block_sums":
+        b = g["cover"] * 4
+    elif name == "k_tile_rows":
+        b = g["cover"] * 4 + g["ntiles"] * g["cx"] * g["T"] * 8
+    elif name == "k_tile_cols":
+        b = g["ntiles"] * g["cx"] * g["T"] * 8 + nsplit * g["cx"] * g["T"] * 4
+    else:
+        return None
+    return b * c
+
+
+def roofline(ctx, ptr, n, a):
+    """One extra pass with every kernel bracketed by HIP events on the
+    library's stream; the dominant HBM-bound kernel's algorithmic bytes per
+    launch over its average launch duration."""
+    g = geometry(a.size, a.pix)
+    ctx.profile(True)
+    ctx.estimate_batch(ptr, n, a.size, a.size, a.pix)
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    chunk = min(4, n)
+    batch_kernels = {"k_tile_reduce"}
+    rows = {}
+    for name, (tot, cnt) in rep.items():
+        per = chunk if name.startswith(("k_rescale", "k_transpose", "k_block", "k_tile_rows",
+                                        "k_tile_cols")) else n
+        if name in batch_kernels:
+            per = chunk
+        avg_ms = tot / cnt
+        nb = kernel_bytes(name, g, per)
+        rows[name] = dict(ms_total=round(tot, 3), launches=cnt, avg_us=round(avg_ms * 1e3, 2),
+                          gbs=None if nb is None else round(nb / (avg_ms * 1e-3) / 1e9, 1))
+    hbm = {k: v for k, v in rows.items() if v["gbs"] is not None}
+    dom = max(hbm, key=lambda k: hbm[k]["ms_total"])
+    per = chunk
+    nb = kernel_bytes(dom, g, per)
+    ach = nb / (rows[dom]["avg_us"] * 1e-6) / 1e9
+    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "algorithmic_bytes_per_launch": int(nb), "avg_launch_us": rows[dom]["avg_us"]}
+    return roof, rows
+
+
+def cpu_baseline(img, pix, gpu_res):
+    """The CPU oracle (one core) on one micrograph of the same workload."""
+    import oracle as O
+    t0 = time.perf_counter()
+    ref, _ = O.estimate(img, pix)
+    dt = time.perf_counter() - t0
+    par = {k: round(abs(gpu_res[k] - ref[k]), 4) for k in ("df_min", "df_max", "azimuth", "score")}
+    return ({"value": round(1.0 / dt, 5), "unit": "micrographs/s", "cores": 1, "kind": "port",
+             "sample": f"1 micrograph {img.shape[1]}x{img.shape[0]} at {pix} A, full plain path "
+                       f"(rescale, spectrum, 1D+2D search), FP64 oracle, {dt:.1f} s"}, par)
 
 
 if __name__ == "__main__":

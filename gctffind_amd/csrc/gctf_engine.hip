@@ -179,11 +179,20 @@ struct gctf_ctx {
 	SearchState* h_pinned = nullptr;
 	int h_pinned_n = 0;
 
+	// kernel timing with HIP events on `stream` (gctf_profile)
+	struct ProfRec { const char* name; hipEvent_t a, b; };
+	bool prof = false;
+	std::vector<ProfRec> prec;
+	std::vector<hipEvent_t> evpool;
+	std::map<std::string, std::pair<double, int>> ptotal;
+
 	~gctf_ctx()
 	{
 		for (auto& kv : graphs)
 			if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
 		if (h_pinned) (void)hipHostFree(h_pinned);
+		for (auto& r : prec) { evpool.push_back(r.a); evpool.push_back(r.b); }
+		for (auto e : evpool) (void)hipEventDestroy(e);
 		if (own_stream) (void)hipStreamDestroy(own_stream);
 	}
 };
@@ -208,6 +217,29 @@ int fail(gctf_ctx* c, int code, const std::string& msg)
 	do {                             \
 		int r_ = (expr);             \
 		if (r_ != GCTF_OK) return r_; \
+	} while (0)
+
+hipEvent_t prof_event(gctf_ctx* c)
+{
+	if (!c->evpool.empty()) { hipEvent_t e = c->evpool.back(); c->evpool.pop_back(); return e; }
+	hipEvent_t e = nullptr;
+	(void)hipEventCreate(&e);
+	return e;
+}
+
+// Launch statement(s) bracketed by events on the context stream when
+// profiling is on.
+#define PROF(c, name, ...)                                                 \
+	do {                                                                   \
+		if ((c)->prof) {                                                   \
+			hipEvent_t a_ = prof_event(c), b_ = prof_event(c);             \
+			(void)hipEventRecord(a_, (c)->stream);                         \
+			__VA_ARGS__;                                                   \
+			(void)hipEventRecord(b_, (c)->stream);                         \
+			(c)->prec.push_back({name, a_, b_});                           \
+		} else {                                                           \
+			__VA_ARGS__;                                                   \
+		}                                                                  \
 	} while (0)
 
 int ensure_buf(gctf_ctx* c, DevBuf& b, size_t bytes)
@@ -517,40 +549,40 @@ int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long lon
 		const int m = rp.rows->m;
 		dim3 grid((rp.ny + 1) / 2, n);
 		Fft1dPlan p = rp.rows->plan();
-		if (m <= 2048) k_rescale_rows_r2c<Tin, 256, 2048><<<grid, 256, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
-		else if (m <= 4096) k_rescale_rows_r2c<Tin, 512, 4096><<<grid, 512, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
-		else if (m <= 8192) k_rescale_rows_r2c<Tin, 1024, 8192><<<grid, 1024, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
-		else k_rescale_rows_r2c<Tin, 1024, 16384><<<grid, 1024, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
+		PROF(c, "k_rescale_rows_r2c", if (m <= 2048) k_rescale_rows_r2c<Tin, 256, 2048><<<grid, 256, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
+			else if (m <= 4096) k_rescale_rows_r2c<Tin, 512, 4096><<<grid, 512, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
+			else if (m <= 8192) k_rescale_rows_r2c<Tin, 1024, 8192><<<grid, 1024, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
+			else k_rescale_rows_r2c<Tin, 1024, 16384><<<grid, 1024, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA));
 	}
 	// 2. transpose (ny x ncx) -> (ncx x ny)
 	{
 		dim3 grid((ncx + 31) / 32, (rp.ny + 31) / 32, n);
-		k_transpose_c<<<grid, 256, 0, s>>>(A, stA, rp.ny, ncx, B, stB);
+		PROF(c, "k_transpose_c", k_transpose_c<<<grid, 256, 0, s>>>(A, stA, rp.ny, ncx, B, stB));
 	}
 	// 3. columns: forward ny, crop, inverse nny
 	{
 		const int m = std::max(rp.cf->m, rp.ci->m);
 		dim3 grid(ncx, n);
 		Fft1dPlan pf = rp.cf->plan(), pi = rp.ci->plan();
-		if (m <= 2048) k_rescale_cols<256, 2048><<<grid, 256, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
-		else if (m <= 4096) k_rescale_cols<512, 4096><<<grid, 512, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
-		else if (m <= 8192) k_rescale_cols<1024, 8192><<<grid, 1024, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
-		else k_rescale_cols<1024, 16384><<<grid, 1024, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
+		PROF(c, "k_rescale_cols", if (m <= 2048) k_rescale_cols<256, 2048><<<grid, 256, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
+			else if (m <= 4096) k_rescale_cols<512, 4096><<<grid, 512, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
+			else if (m <= 8192) k_rescale_cols<1024, 8192><<<grid, 1024, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
+			else k_rescale_cols<1024, 16384><<<grid, 1024, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA));
 	}
 	// 4. transpose (ncx x nny) -> (nny x ncx)
 	{
 		dim3 grid((rp.nny + 31) / 32, (ncx + 31) / 32, n);
-		k_transpose_c<<<grid, 256, 0, s>>>(A, stA, ncx, rp.nny, B, stB);
+		PROF(c, "k_transpose_c", k_transpose_c<<<grid, 256, 0, s>>>(A, stA, ncx, rp.nny, B, stB));
 	}
 	// 5. rows C2R into the padded image
 	{
 		const int m = rp.c2r->m;
 		dim3 grid((rp.nny + 1) / 2, n);
 		Fft1dPlan p = rp.c2r->plan();
-		if (m <= 2048) k_rescale_rows_c2r<256, 2048><<<grid, 256, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
-		else if (m <= 4096) k_rescale_rows_c2r<512, 4096><<<grid, 512, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
-		else if (m <= 8192) k_rescale_rows_c2r<1024, 8192><<<grid, 1024, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
-		else k_rescale_rows_c2r<1024, 16384><<<grid, 1024, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
+		PROF(c, "k_rescale_rows_c2r", if (m <= 2048) k_rescale_rows_c2r<256, 2048><<<grid, 256, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
+			else if (m <= 4096) k_rescale_rows_c2r<512, 4096><<<grid, 512, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
+			else if (m <= 8192) k_rescale_rows_c2r<1024, 8192><<<grid, 1024, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
+			else k_rescale_rows_c2r<1024, 16384><<<grid, 1024, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch));
 	}
 	HIPCHK(c, hipGetLastError());
 	return GCTF_OK;
@@ -593,19 +625,19 @@ int launch_spectrum_t(gctf_ctx* c, const TileGrid& g, const TileTables& tt, cons
 	constexpr int CX = T / 2 + 1;
 	const int ntiles = g.ntx * g.nty;
 	const int nsplit = std::min(c->tile_splits, ntiles);
-	k_block_sums<Tin><<<dim3(g.nbx * g.nby, n), 256, 0, s>>>(img, stride, pitch, g, c->bsums.as<double>());
+	PROF(c, "k_block_sums", k_block_sums<Tin><<<dim3(g.nbx * g.nby, n), 256, 0, s>>>(img, stride, pitch, g, c->bsums.as<double>()));
 	constexpr int ROWS = 16;
 	const long long rts = (long long)ntiles * CX * T;
-	k_tile_rows<Tin, T, ROWS><<<dim3(ntiles * (T / ROWS), n), (ROWS / 2) * (T / 8), 0, s>>>(
+	PROF(c, "k_tile_rows", k_tile_rows<Tin, T, ROWS><<<dim3(ntiles * (T / ROWS), n), (ROWS / 2) * (T / 8), 0, s>>>(
 		img, stride, pitch, g, c->bsums.as<double>(), tt.mask.as<float>(), tt.tw.as<float2>(),
-		c->rt.as<float2>(), rts);
+		c->rt.as<float2>(), rts));
 	constexpr int CPW = 4;
 	const float fac = 1.0f / (float)ntiles;                   // CGenAvgSpectrum.cpp:90
-	k_tile_cols<T, CPW><<<dim3((CX + CPW - 1) / CPW, nsplit, n), CPW * (T / 8), 0, s>>>(
-		c->rt.as<float2>(), rts, ntiles, nsplit, fac, tt.tw.as<float2>(), c->tpart.as<float>());
+	PROF(c, "k_tile_cols", k_tile_cols<T, CPW><<<dim3((CX + CPW - 1) / CPW, nsplit, n), CPW * (T / 8), 0, s>>>(
+		c->rt.as<float2>(), rts, ntiles, nsplit, fac, tt.tw.as<float2>(), c->tpart.as<float>()));
 	const int npix = T * CX;
-	k_tile_reduce<<<dim3((npix + 255) / 256, n), 256, 0, s>>>(c->tpart.as<float>(), nsplit, npix,
-		c->raw.as<float>() + (size_t)slot0 * npix);
+	PROF(c, "k_tile_reduce", k_tile_reduce<<<dim3((npix + 255) / 256, n), 256, 0, s>>>(c->tpart.as<float>(), nsplit, npix,
+		c->raw.as<float>() + (size_t)slot0 * npix));
 	HIPCHK(c, hipGetLastError());
 	return GCTF_OK;
 }
@@ -637,15 +669,15 @@ int launch_post_t(gctf_ctx* c, const TileTables& tt, int n)
 	if (box < 7) box = 7;
 	const int hb = box / 2;
 	const int ls = c->prm.log_spect ? 1 : 0;
-	k_bg_rows<T><<<dim3(T, n), 256, 0, s>>>(c->raw.as<float>(), ls, hb, c->hsum.as<float>());
+	PROF(c, "k_bg_rows", k_bg_rows<T><<<dim3(T, n), 256, 0, s>>>(c->raw.as<float>(), ls, hb, c->hsum.as<float>()));
 	const int nparts = (npix + 255) / 256;
-	k_bg_cols<T><<<dim3(nparts, 1, n), 256, 0, s>>>(c->raw.as<float>(), c->hsum.as<float>(), ls, hb,
-		c->spect.as<float>(), c->mom.as<double>());
-	k_gate<<<n, 256, 0, s>>>(c->mom.as<double>(), nparts, npix, c->states.as<SearchState>());
-	k_lp_x<T><<<dim3(T / 4, n), 2 * (T / 8), 0, s>>>(c->spect.as<float>(), c->states.as<SearchState>(),
-		tt.h1.as<float>(), tt.tw.as<float2>(), c->q.as<float>());
-	k_lp_y<T><<<dim3((CX + 1) / 2, n), T / 8, 0, s>>>(c->q.as<float>(), c->states.as<SearchState>(),
-		tt.h1.as<float>(), tt.tw.as<float2>(), c->spect.as<float>());
+	PROF(c, "k_bg_cols", k_bg_cols<T><<<dim3(nparts, 1, n), 256, 0, s>>>(c->raw.as<float>(), c->hsum.as<float>(), ls, hb,
+		c->spect.as<float>(), c->mom.as<double>()));
+	PROF(c, "k_gate", k_gate<<<n, 256, 0, s>>>(c->mom.as<double>(), nparts, npix, c->states.as<SearchState>()));
+	PROF(c, "k_lp_x", k_lp_x<T><<<dim3(T / 4, n), 2 * (T / 8), 0, s>>>(c->spect.as<float>(), c->states.as<SearchState>(),
+		tt.h1.as<float>(), tt.tw.as<float2>(), c->q.as<float>()));
+	PROF(c, "k_lp_y", k_lp_y<T><<<dim3((CX + 1) / 2, n), T / 8, 0, s>>>(c->q.as<float>(), c->states.as<SearchState>(),
+		tt.h1.as<float>(), tt.tw.as<float2>(), c->spect.as<float>()));
 	HIPCHK(c, hipGetLastError());
 	return GCTF_OK;
 }
@@ -666,12 +698,12 @@ int launch_1d(gctf_ctx* c, const SearchPlan& sp, const TileTables& tt, int n, in
 {
 	hipStream_t s = c->stream;
 	const int T = sp.T, cx = T / 2 + 1;
-	k_radial<<<n, 256, 0, s>>>(c->spect.as<float>(), cx, T, tt.roff.as<int>(), tt.rcnt.as<int>(),
-		tt.ridx.as<int>(), c->radial.as<float>());
+	PROF(c, "k_radial", k_radial<<<n, 256, 0, s>>>(c->spect.as<float>(), cx, T, tt.roff.as<int>(), tt.rcnt.as<int>(),
+		tt.ridx.as<int>(), c->radial.as<float>()));
 	Search1DParams p;
 	p.c = sp.cc; p.cmpx = cx; p.lo = sp.lo1d; p.hi = sp.hi1d;
 	p.mode = mode; p.init_df = init_df; p.range = range;
-	k_search1d<<<n, 256, 0, s>>>(p, c->radial.as<float>(), tt.w1d.as<double>(), c->states.as<SearchState>());
+	PROF(c, "k_search1d", k_search1d<<<n, 256, 0, s>>>(p, c->radial.as<float>(), tt.w1d.as<double>(), c->states.as<SearchState>()));
 	HIPCHK(c, hipGetLastError());
 	return GCTF_OK;
 }
@@ -684,15 +716,15 @@ int launch_2d(gctf_ctx* c, const SearchPlan& sp, int n, int begin, float phase_r
 	SearchState* st = c->states.as<SearchState>();
 	double2* part = c->part2d.as<double2>();
 	const double* ss = c->ring_ss.as<double>();
-	k_ring_gather<<<n, 1024, 0, s>>>(c->spect.as<float>(), T * cx, sp.ring, c->ring_spec.as<float>(),
-		c->ring_ss.as<double>());
+	PROF(c, "k_ring_gather", k_ring_gather<<<n, 1024, 0, s>>>(c->spect.as<float>(), T * cx, sp.ring, c->ring_spec.as<float>(),
+		c->ring_ss.as<double>()));
 	constexpr int CPB = 8;
 	auto eval = [&](int maxc) {
 		dim3 grid((maxc + CPB - 1) / CPB, kPixChunks, n);
-		k_cc2d_eval<CPB><<<grid, 256, 0, s>>>(sp.cc, sp.ring, c->ring_spec.as<float>(), st, part);
+		PROF(c, "k_cc2d_eval", k_cc2d_eval<CPB><<<grid, 256, 0, s>>>(sp.cc, sp.ring, c->ring_spec.as<float>(), st, part));
 	};
 	auto step = [&](int next) {
-		k_s2d_step<<<n, 256, 0, s>>>(st, part, ss, next, c->refine.as<RefineArgs>());
+		PROF(c, "k_s2d_step", k_s2d_step<<<n, 256, 0, s>>>(st, part, ss, next, c->refine.as<RefineArgs>()));
 	};
 	int maxph = 61;
 	if (phase_range > 61.0f) maxph = std::min(kMaxCands, (int)phase_range + 3);
@@ -711,9 +743,9 @@ int launch_2d(gctf_ctx* c, const SearchPlan& sp, int n, int begin, float phase_r
 		if (phase_range > 0.0f) eval(maxph);
 		step(i < 19 ? NX_ASTMAG : NX_FINISH);
 	}
-	k_spectral_cc<<<dim3(cx, n), 256, 0, s>>>(sp.cc, cx, T, 10, c->spect.as<float>(), st,
-		c->shell_cc.as<float>());
-	k_ctf_res<<<(n + 63) / 64, 64, 0, s>>>(c->shell_cc.as<float>(), cx, T, sp.pix, st, n);
+	PROF(c, "k_spectral_cc", k_spectral_cc<<<dim3(cx, n), 256, 0, s>>>(sp.cc, cx, T, 10, c->spect.as<float>(), st,
+		c->shell_cc.as<float>()));
+	PROF(c, "k_ctf_res", k_ctf_res<<<(n + 63) / 64, 64, 0, s>>>(c->shell_cc.as<float>(), cx, T, sp.pix, st, n));
 	HIPCHK(c, hipGetLastError());
 	return GCTF_OK;
 }
@@ -886,6 +918,46 @@ int gctf_tune(gctf_ctx* ctx, int chunk, int tile_splits, int use_graph)
 	if (chunk > 0) ctx->chunk = chunk;
 	if (tile_splits > 0) { ctx->tile_splits = tile_splits; ctx->gen++; }
 	if (use_graph != 0) ctx->use_graph = use_graph > 0 ? 1 : 0;
+	return GCTF_OK;
+}
+
+int gctf_profile(gctf_ctx* ctx, int enable)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	for (auto& r : ctx->prec) { ctx->evpool.push_back(r.a); ctx->evpool.push_back(r.b); }
+	ctx->prec.clear();
+	ctx->ptotal.clear();
+	ctx->prof = enable != 0;
+	return GCTF_OK;
+}
+
+int gctf_profile_report(gctf_ctx* ctx, int max, char* names, double* total_ms, int* count, int* nout)
+{
+	if (!ctx || !nout) return GCTF_ERR_ARG;
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	for (auto& r : ctx->prec) {
+		float ms = 0.f;
+		HIPCHK(ctx, hipEventElapsedTime(&ms, r.a, r.b));
+		auto& t = ctx->ptotal[r.name];
+		t.first += ms;
+		t.second += 1;
+		ctx->evpool.push_back(r.a);
+		ctx->evpool.push_back(r.b);
+	}
+	ctx->prec.clear();
+	int i = 0;
+	for (auto& kv : ctx->ptotal) {
+		if (i >= max) break;
+		if (names) {
+			strncpy(names + 64 * i, kv.first.c_str(), 63);
+			names[64 * i + 63] = 0;
+		}
+		if (total_ms) total_ms[i] = kv.second.first;
+		if (count) count[i] = kv.second.second;
+		++i;
+	}
+	*nout = i;
 	return GCTF_OK;
 }
 
@@ -1204,7 +1276,7 @@ int gctf_estimate_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, i
 		CHK(ensure_rescale_buffers(ctx, rp, C));
 		CHK(ensure_buf(ctx, ctx->padc, sizeof(float) * (size_t)rp.pitch * rp.nny * C));
 	}
-	if (!ctx->use_graph) {
+	if (!ctx->use_graph || ctx->prof) {
 		CHK(record_batch(ctx, rp, *sp, *tt, dev_imgs, dtype, n, stride));
 		return read_results(ctx, n, out);
 	}

@@ -25,7 +25,8 @@ EXPORTS = [
     "gctf_rescale_setup", "gctf_rescale", "gctf_scaled_image", "gctf_setup1", "gctf_setup2",
     "gctf_set_phase", "gctf_gen_half_spectrum", "gctf_do1d", "gctf_refine1d", "gctf_do2d",
     "gctf_refine", "gctf_spect_size", "gctf_get_half_spect", "gctf_set_half_spect",
-    "gctf_full_spectrum", "gctf_estimate_batch", "gctf_tune",
+    "gctf_full_spectrum", "gctf_estimate_batch", "gctf_tune", "gctf_profile",
+    "gctf_profile_report",
 ]
 
 
@@ -87,6 +88,8 @@ def load():
         "gctf_full_spectrum": (ci, [vp, fp]),
         "gctf_estimate_batch": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, fp]),
         "gctf_tune": (ci, [vp, ci, ci, ci]),
+        "gctf_profile": (ci, [vp, ci]),
+        "gctf_profile_report": (ci, [vp, ci, fp, fp, fp, fp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -213,6 +216,22 @@ class Context:
 
     def tune(self, chunk=0, tile_splits=0, use_graph=0):
         self.check(self.L.gctf_tune(self.h, chunk, tile_splits, use_graph))
+
+    def profile(self, enable=True):
+        self.check(self.L.gctf_profile(self.h, int(enable)))
+
+    def profile_report(self, max_rows=64):
+        names = ctypes.create_string_buffer(64 * max_rows)
+        tot = np.zeros(max_rows, np.float64)
+        cnt = np.zeros(max_rows, np.int32)
+        n = ctypes.c_int()
+        self.check(self.L.gctf_profile_report(self.h, max_rows, ctypes.cast(names, ctypes.c_void_p),
+                                              _np_ptr(tot), _np_ptr(cnt), ctypes.cast(ctypes.pointer(n), ctypes.c_void_p)))
+        out = {}
+        for i in range(n.value):
+            nm = names.raw[64 * i: 64 * (i + 1)].split(b"\0")[0].decode()
+            out[nm] = (float(tot[i]), int(cnt[i]))
+        return out
 
     def synchronize(self):
         self.check(self.L.gctf_synchronize(self.h))

@@ -123,6 +123,15 @@ GCTF_API int gctf_estimate_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype,
  * spectrum chunk, tile-column splits, graph use (1 on, -1 off). */
 GCTF_API int gctf_tune(gctf_ctx* ctx, int chunk, int tile_splits, int use_graph);
 
+/* Kernel timing (no reference counterpart): while enabled, every kernel the
+ * context launches is bracketed by HIP events on its stream and batches run
+ * without graphs.  gctf_profile(ctx, 1) starts and clears; the report
+ * aggregates per kernel name since the start: names (64 chars each),
+ * total milliseconds and launch counts, at most `max` rows. */
+GCTF_API int gctf_profile(gctf_ctx* ctx, int enable);
+GCTF_API int gctf_profile_report(gctf_ctx* ctx, int max, char* names, double* total_ms,
+	int* count, int* nout);
+
 #ifdef __cplusplus
 }
 #endif
