@@ -35,7 +35,8 @@ def parse():
     ap.add_argument("--pix", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--check", type=int, default=2, help="micrographs checked against the oracle")
+    ap.add_argument("--chunk", type=int, default=0, help="images per spectrum chunk (0: library default)")
+    ap.add_argument("--splits", type=int, default=0, help="tile-column splits (0: library default)")
     return ap.parse_args()
 
 
@@ -66,6 +67,7 @@ def main():
     from gctffind_amd import gctf
 
     ctx = gctf.Context()
+    ctx.tune(chunk=a.chunk, tile_splits=a.splits)
     imgs, truth = make_batch(torch, a.batch, a.size, a.pix)
     n = a.batch
     ptr = imgs.data_ptr()
@@ -112,12 +114,14 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def geometry(size, pix, tile=512):
-    """Shapes of the plain path for a size x size image (CRescaleImage /
-    CGenAvgSpectrum rules, as the oracle computes them)."""
-    import oracle as O_  # shape rules only (integer/float32 arithmetic)
-    nnx, nny, pitch, pixn, b = O_.rescale_sizes(size, size, pix)
-    ntx, nty, ox, oy, ov, step = O_.tile_grid(nnx, nny, tile)
+def geometry(ctx, size, pix, tile=512):
+    """Shapes of the plain path for a size x size image: the rescaled size from
+    the library (CRescaleImage::Setup) and the tile grid of
+    CGenAvgSpectrum::SetSizes (integer rules)."""
+    nnx, nny, pitch, pixn = ctx.rescale_setup(size, size, pix)
+    ov = int(tile * 0.5) // 2 * 2
+    step = tile - ov
+    ntx, nty = (nnx - ov) // step, (nny - ov) // step
     return dict(nx=size, ny=size, nnx=nnx, nny=nny, pitch=pitch, ncx=nnx // 2 + 1, ntiles=ntx * nty,
                 cover=((ntx + 1) * step) * ((nty + 1) * step), cx=tile // 2 + 1, T=tile)
 
@@ -149,12 +153,12 @@ def roofline(ctx, ptr, n, a):
     """One extra pass with every kernel bracketed by HIP events on the
     library's stream; the dominant HBM-bound kernel's algorithmic bytes per
     launch over its average launch duration."""
-    g = geometry(a.size, a.pix)
+    g = geometry(ctx, a.size, a.pix)
     ctx.profile(True)
     ctx.estimate_batch(ptr, n, a.size, a.size, a.pix)
     rep = ctx.profile_report()
     ctx.profile(False)
-    chunk = min(4, n)
+    chunk = min(a.chunk or 4, n)
     batch_kernels = {"k_tile_reduce"}
     rows = {}
     for name, (tot, cnt) in rep.items():

@@ -609,7 +609,7 @@ int ensure_spectrum_buffers(gctf_ctx* c, const TileGrid& g, int T, int n)
 {
 	const size_t cx = T / 2 + 1;
 	const size_t ntiles = (size_t)g.ntx * g.nty;
-	CHK(ensure_buf(c, c->bsums, sizeof(double) * (size_t)g.nbx * g.nby * n));
+	CHK(ensure_buf(c, c->bsums, sizeof(double) * (size_t)g.nbx * g.nby * kSumParts * n));
 	CHK(ensure_buf(c, c->rt, sizeof(float2) * ntiles * cx * T * n));
 	CHK(ensure_buf(c, c->tpart, sizeof(float) * (size_t)c->tile_splits * T * cx * n));
 	return GCTF_OK;
@@ -625,7 +625,7 @@ int launch_spectrum_t(gctf_ctx* c, const TileGrid& g, const TileTables& tt, cons
 	constexpr int CX = T / 2 + 1;
 	const int ntiles = g.ntx * g.nty;
 	const int nsplit = std::min(c->tile_splits, ntiles);
-	PROF(c, "k_block_sums", k_block_sums<Tin><<<dim3(g.nbx * g.nby, n), 256, 0, s>>>(img, stride, pitch, g, c->bsums.as<double>()));
+	PROF(c, "k_block_sums", k_block_sums<Tin><<<dim3(g.nbx * g.nby, kSumParts, n), 256, 0, s>>>(img, stride, pitch, g, c->bsums.as<double>()));
 	constexpr int ROWS = 16;
 	const long long rts = (long long)ntiles * CX * T;
 	PROF(c, "k_tile_rows", k_tile_rows<Tin, T, ROWS><<<dim3(ntiles * (T / ROWS), n), (ROWS / 2) * (T / 8), 0, s>>>(

@@ -7,8 +7,14 @@
 // on a power-of-two length M >= 2n-1.  Both are unnormalised; the forward
 // transform uses exp(-2 pi i k n / N).
 //
-// Twiddle tables (tw[e] = exp(-2 pi i e / N), e < N) and Bluestein chirps
-// are computed on the host in double and rounded once to float.
+// LDS layout: logical element i lives at physical index pidx(i) = i + i/8.
+// The Stockham writes of the early passes stride by the radix (8 complex);
+// the odd multiplier 9 spreads those strides over all banks.  A buffer of N
+// logical elements therefore needs fft_lds_len(N) = N + N/8 entries.
+//
+// Twiddles: one load w = exp(-2 pi i k S / N) per butterfly from a table
+// computed on the host in double; the powers w^2 .. w^(R-1) are formed by
+// complex products of depth <= 3.
 //
 // All code here is compiled with -ffp-contract=off; fused multiply-adds
 // are written out with fmaf where they are wanted.
@@ -16,6 +22,9 @@
 #include <hip/hip_runtime.h>
 
 namespace gctf {
+
+__host__ __device__ constexpr int fft_lds_len(int n) { return n + n / 8; }
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 3); }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
@@ -91,6 +100,22 @@ __device__ __forceinline__ void dft_small<8, false>(float2* u) { dft8<false>(u);
 template <>
 __device__ __forceinline__ void dft_small<8, true>(float2* u) { dft8<true>(u); }
 
+// u[r] *= w^r for r = 1 .. R-1 (w already conjugated for the inverse).
+template <int R>
+__device__ __forceinline__ void twiddle_powers(float2* u, float2 w)
+{
+	float2 p[R];
+	p[1] = w;
+#pragma unroll
+	for (int r = 2; r < R; ++r) {
+		int hb = 1;
+		while (hb * 2 <= r) hb *= 2;
+		p[r] = (hb == r) ? cmul(p[r / 2], p[r / 2]) : cmul(p[hb], p[r - hb]);
+	}
+#pragma unroll
+	for (int r = 1; r < R; ++r) u[r] = cmul(u[r], p[r]);
+}
+
 // One Stockham pass of radix R over N points; L = product of the radices of
 // the previous passes.  NT threads (tid in [0, NT)) share the buffer.
 template <int N, int R, int L, int NT, bool INV>
@@ -104,16 +129,13 @@ __device__ __forceinline__ void fft_pass(float2* buf, const float2* __restrict__
 		const int j = tid + b * NT;
 		if (T % NT == 0 || j < T) {
 #pragma unroll
-			for (int r = 0; r < R; ++r) u[b][r] = buf[j + r * T];
+			for (int r = 0; r < R; ++r) u[b][r] = buf[pidx(j + r * T)];
 			if constexpr (L > 1) {
 				const int k = j & (L - 1);
 				constexpr int S = N / (L * R);
-#pragma unroll
-				for (int r = 1; r < R; ++r) {
-					float2 w = tw[(r * k) * S];
-					if (INV) w.y = -w.y;
-					u[b][r] = cmul(u[b][r], w);
-				}
+				float2 w = tw[k * S];
+				if (INV) w.y = -w.y;
+				twiddle_powers<R>(u[b], w);
 			}
 			dft_small<R, INV>(u[b]);
 		}
@@ -126,7 +148,7 @@ __device__ __forceinline__ void fft_pass(float2* buf, const float2* __restrict__
 			const int k = j & (L - 1);
 			const int base = (j - k) * R + k;
 #pragma unroll
-			for (int r = 0; r < R; ++r) buf[base + r * L] = u[b][r];
+			for (int r = 0; r < R; ++r) buf[pidx(base + r * L)] = u[b][r];
 		}
 	}
 	__syncthreads();
@@ -143,9 +165,10 @@ __device__ __forceinline__ void fft_run(float2* buf, const float2* __restrict__ 
 	}
 }
 
-// In-place power-of-two FFT of buf[0..N) in LDS.  Every thread of the
-// workgroup must call it (it contains __syncthreads); the data must be in
-// buf and visible (barrier) before the call; results are visible after.
+// In-place power-of-two FFT of the N logical elements of buf (padded
+// layout, fft_lds_len(N) entries).  Every thread of the workgroup must call
+// it (it contains __syncthreads); the data must be in buf and visible
+// (barrier) before the call; results are visible after.
 template <int N, int NT, bool INV>
 __device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw, int tid)
 {
@@ -164,15 +187,12 @@ __device__ __forceinline__ void fft_pass_rt(float2* buf, const float2* __restric
 		const int j = tid + b * NT;
 		if (j < T) {
 #pragma unroll
-			for (int r = 0; r < R; ++r) u[b][r] = buf[j + r * T];
+			for (int r = 0; r < R; ++r) u[b][r] = buf[pidx(j + r * T)];
 			if (L > 1) {
 				const int k = j & (L - 1);
-#pragma unroll
-				for (int r = 1; r < R; ++r) {
-					float2 w = tw[(r * k) * S];
-					if (INV) w.y = -w.y;
-					u[b][r] = cmul(u[b][r], w);
-				}
+				float2 w = tw[k * S];
+				if (INV) w.y = -w.y;
+				twiddle_powers<R>(u[b], w);
 			}
 			dft_small<R, INV>(u[b]);
 		}
@@ -185,7 +205,7 @@ __device__ __forceinline__ void fft_pass_rt(float2* buf, const float2* __restric
 			const int k = j & (L - 1);
 			const int base = (j - k) * R + k;
 #pragma unroll
-			for (int r = 0; r < R; ++r) buf[base + r * L] = u[b][r];
+			for (int r = 0; r < R; ++r) buf[pidx(base + r * L)] = u[b][r];
 		}
 	}
 	__syncthreads();
@@ -212,10 +232,11 @@ __device__ __forceinline__ void fft_lds_rt(float2* buf, int n, const float2* __r
 	}
 }
 
-// Bluestein DFT of length n held in buf[0..n); buf has m entries (m a power
-// of two >= 2n-1).  chirp[k] = exp(-pi i (k^2 mod 2n) / n); bhat = FFT_m of
-// b[j] = conj(chirp[|j|]) (wrapped).  twm: the m-point twiddle table.
-// The inverse uses conj(chirp) and bhat_inv[j] = conj(bhat[(m-j) mod m]).
+// Bluestein DFT of length n held in logical buf[0..n); buf holds m logical
+// entries (m a power of two >= 2n-1).  chirp[k] = exp(-pi i (k^2 mod 2n) / n);
+// bhat = FFT_m of b[j] = conj(chirp[|j|]) (wrapped).  twm: the m-point
+// twiddle table.  The inverse uses conj(chirp) and
+// bhat_inv[j] = conj(bhat[(m-j) mod m]).
 template <int NT, int MAXN, bool INV>
 __device__ __forceinline__ void bluestein_lds(float2* buf, int n, int m, const float2* __restrict__ chirp,
 	const float2* __restrict__ bhat, const float2* __restrict__ twm, int tid)
@@ -224,9 +245,9 @@ __device__ __forceinline__ void bluestein_lds(float2* buf, int n, int m, const f
 		if (k < n) {
 			float2 w = chirp[k];
 			if (INV) w.y = -w.y;
-			buf[k] = cmul(buf[k], w);
+			buf[pidx(k)] = cmul(buf[pidx(k)], w);
 		} else {
-			buf[k] = make_float2(0.f, 0.f);
+			buf[pidx(k)] = make_float2(0.f, 0.f);
 		}
 	}
 	__syncthreads();
@@ -239,7 +260,7 @@ __device__ __forceinline__ void bluestein_lds(float2* buf, int n, int m, const f
 		} else {
 			b = bhat[k];
 		}
-		buf[k] = cmul(buf[k], b);
+		buf[pidx(k)] = cmul(buf[pidx(k)], b);
 	}
 	__syncthreads();
 	fft_lds_rt<NT, MAXN, true>(buf, m, twm, tid);
@@ -247,8 +268,8 @@ __device__ __forceinline__ void bluestein_lds(float2* buf, int n, int m, const f
 	for (int k = tid; k < n; k += NT) {
 		float2 w = chirp[k];
 		if (INV) w.y = -w.y;
-		float2 v = cmul(buf[k], w);
-		buf[k] = make_float2(v.x * inv_m, v.y * inv_m);
+		float2 v = cmul(buf[pidx(k)], w);
+		buf[pidx(k)] = make_float2(v.x * inv_m, v.y * inv_m);
 	}
 	__syncthreads();
 }

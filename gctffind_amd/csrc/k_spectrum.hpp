@@ -20,7 +20,7 @@ __global__ __launch_bounds__(NT) void k_rescale_rows_r2c(const Tin* __restrict__
 	long long img_stride, int nx, int ny, Fft1dPlan plan, int ncx_out,
 	float2* __restrict__ out, long long out_stride)
 {
-	__shared__ float2 buf[MAXN];
+	__shared__ float2 buf[fft_lds_len(MAXN)];
 	const int tid = threadIdx.x;
 	const int y0 = 2 * blockIdx.x;
 	const bool two = (y0 + 1) < ny;
@@ -28,14 +28,14 @@ __global__ __launch_bounds__(NT) void k_rescale_rows_r2c(const Tin* __restrict__
 	for (int x = tid; x < nx; x += NT) {
 		float a = load_px(src, x);
 		float b = two ? load_px(src, (long long)nx + x) : 0.f;
-		buf[x] = make_float2(a, b);
+		buf[pidx(x)] = make_float2(a, b);
 	}
 	__syncthreads();
 	fft_any_lds<NT, MAXN, false>(buf, plan, tid);
 	float2* dst = out + blockIdx.y * out_stride + (long long)y0 * ncx_out;
 	for (int k = tid; k < ncx_out; k += NT) {
-		float2 z = buf[k];
-		float2 zc = buf[k == 0 ? 0 : nx - k];
+		float2 z = buf[pidx(k)];
+		float2 zc = buf[pidx(k == 0 ? 0 : nx - k)];
 		// A = (Z[k] + conj Z[N-k]) / 2 ; B = -i (Z[k] - conj Z[N-k]) / 2
 		float2 A = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
 		float2 B = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
@@ -72,10 +72,10 @@ __global__ __launch_bounds__(NT) void k_rescale_cols(const float2* __restrict__ 
 	long long in_stride, int ny, int nny, Fft1dPlan pf, Fft1dPlan pi,
 	float2* __restrict__ out, long long out_stride)
 {
-	__shared__ float2 buf[MAXN];
+	__shared__ float2 buf[fft_lds_len(MAXN)];
 	const int tid = threadIdx.x;
 	const float2* src = in + blockIdx.y * in_stride + (long long)blockIdx.x * ny;
-	for (int y = tid; y < ny; y += NT) buf[y] = src[y];
+	for (int y = tid; y < ny; y += NT) buf[pidx(y)] = src[y];
 	__syncthreads();
 	fft_any_lds<NT, MAXN, false>(buf, pf, tid);
 	// crop: gather into registers, then rewrite the head of the buffer
@@ -86,19 +86,19 @@ __global__ __launch_bounds__(NT) void k_rescale_cols(const float2* __restrict__ 
 		int y = tid + i * NT;
 		if (y < nny) {
 			int ys = y > nny / 2 ? y - nny + ny : y;
-			v[i] = buf[ys];
+			v[i] = buf[pidx(ys)];
 		}
 	}
 	__syncthreads();
 #pragma unroll
 	for (int i = 0; i < PER; ++i) {
 		int y = tid + i * NT;
-		if (y < nny) buf[y] = v[i];
+		if (y < nny) buf[pidx(y)] = v[i];
 	}
 	__syncthreads();
 	fft_any_lds<NT, MAXN, true>(buf, pi, tid);
 	float2* dst = out + blockIdx.y * out_stride + (long long)blockIdx.x * nny;
-	for (int y = tid; y < nny; y += NT) dst[y] = buf[y];
+	for (int y = tid; y < nny; y += NT) dst[y] = buf[pidx(y)];
 }
 
 // Half-spectrum rows -> real rows of length nnx (C2R; imaginary parts of the
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(NT) void k_rescale_rows_c2r(const float2* __restric
 	long long in_stride, int nny, int nnx, Fft1dPlan plan, float* __restrict__ out,
 	long long out_stride, int pitch)
 {
-	__shared__ float2 buf[MAXN];
+	__shared__ float2 buf[fft_lds_len(MAXN)];
 	const int tid = threadIdx.x;
 	const int ncx = nnx / 2 + 1;
 	const int y0 = 2 * blockIdx.x;
@@ -129,13 +129,13 @@ __global__ __launch_bounds__(NT) void k_rescale_rows_c2r(const float2* __restric
 			b.y = -b.y;
 		}
 		// Z = A + i B
-		buf[k] = make_float2(a.x - b.y, a.y + b.x);
+		buf[pidx(k)] = make_float2(a.x - b.y, a.y + b.x);
 	}
 	__syncthreads();
 	fft_any_lds<NT, MAXN, true>(buf, plan, tid);
 	float* da = out + blockIdx.y * out_stride + (long long)y0 * pitch;
 	for (int x = tid; x < pitch; x += NT) {
-		float2 z = x < nnx ? buf[x] : make_float2(0.f, 0.f);
+		float2 z = x < nnx ? buf[pidx(x)] : make_float2(0.f, 0.f);
 		da[x] = z.x;
 		if (two) da[pitch + x] = z.y;
 	}
@@ -153,31 +153,44 @@ struct TileGrid {
 	int bsize;           // block side (step, or tile when blocks == 0)
 };
 
-// FP64 sums of bsize x bsize blocks at (offx + i*step, offy + j*step).
-// Grid (nbx*nby, images), 256 threads.
+// FP64 sums of bsize x bsize blocks at (offx + i*step, offy + j*step), each
+// block split over kSumParts row groups.  Grid (nbx*nby, kSumParts, images),
+// 256 threads; sums[image][block][part].
+constexpr int kSumParts = 8;
+
 template <typename Tin>
 __global__ __launch_bounds__(256) void k_block_sums(const Tin* __restrict__ img,
 	long long img_stride, int pitch, TileGrid g, double* __restrict__ sums)
 {
 	__shared__ double red[256];
 	const int bi = blockIdx.x % g.nbx, bj = blockIdx.x / g.nbx;
+	const int rows = (g.bsize + kSumParts - 1) / kSumParts;
+	const int y0 = blockIdx.y * rows, y1 = min(g.bsize, y0 + rows);
 	const int sx = g.offx + bi * g.step, sy = g.offy + bj * g.step;
-	const Tin* src = img + blockIdx.y * img_stride;
+	const Tin* src = img + blockIdx.z * img_stride;
 	const int tid = threadIdx.x;
-	double acc = 0.0;
-	for (int y = 0; y < g.bsize; ++y) {
-		const Tin* row = src + (long long)(sy + y) * pitch + sx;
-		float a = 0.f;
-		for (int x = tid; x < g.bsize; x += 256) a += load_px(row, x);
-		acc += (double)a;
+	float a = 0.f;
+	const int n = (y1 - y0) * g.bsize;
+	for (int i = tid; i < n; i += 256) {
+		const int y = y0 + i / g.bsize, x = i % g.bsize;
+		a += load_px(src, (long long)(sy + y) * pitch + sx + x);
 	}
-	red[tid] = acc;
+	red[tid] = (double)a;
 	__syncthreads();
 	for (int s = 128; s > 0; s >>= 1) {
 		if (tid < s) red[tid] += red[tid + s];
 		__syncthreads();
 	}
-	if (tid == 0) sums[(long long)blockIdx.y * g.nbx * g.nby + blockIdx.x] = red[0];
+	if (tid == 0)
+		sums[((long long)blockIdx.z * g.nbx * g.nby + blockIdx.x) * kSumParts + blockIdx.y] = red[0];
+}
+
+__device__ __forceinline__ double block_sum(const double* bs, int b)
+{
+	double s = 0.0;
+#pragma unroll
+	for (int p = 0; p < kSumParts; ++p) s += bs[b * kSumParts + p];
+	return s;
 }
 
 // Row pass of the tile FFT: for ROWS rows of one tile, subtract the tile
@@ -195,36 +208,37 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8)) void k_tile_rows(const Tin* _
 	constexpr int NG = ROWS / 2;           // transforms per workgroup
 	constexpr int NT = NG * FT;
 	constexpr int CX = T / 2 + 1;
-	__shared__ float2 buf[NG * T];
+	constexpr int FL = fft_lds_len(T);
+	__shared__ float2 buf[NG * FL];
 	__shared__ float2 stage[CX * ROWS];
 	const int rb = blockIdx.x % (T / ROWS);
 	const int t = blockIdx.x / (T / ROWS);
 	const int tx = t % g.ntx, ty = t / g.ntx;
 	const int sx = g.offx + tx * g.step, sy = g.offy + ty * g.step;
-	const long long nb = (long long)g.nbx * g.nby;
+	const long long nb = (long long)g.nbx * g.nby * kSumParts;
 	const double* bs = sums + blockIdx.y * nb;
 	double tsum;
 	if (g.blocks) {
-		tsum = bs[ty * g.nbx + tx] + bs[ty * g.nbx + tx + 1] + bs[(ty + 1) * g.nbx + tx]
-			+ bs[(ty + 1) * g.nbx + tx + 1];
+		tsum = block_sum(bs, ty * g.nbx + tx) + block_sum(bs, ty * g.nbx + tx + 1)
+			+ block_sum(bs, (ty + 1) * g.nbx + tx) + block_sum(bs, (ty + 1) * g.nbx + tx + 1);
 	} else {
-		tsum = bs[ty * g.nbx + tx];
+		tsum = block_sum(bs, ty * g.nbx + tx);
 	}
 	const float mean = (float)(tsum / ((double)T * (double)T));
 	const int grp = threadIdx.x / FT, lane = threadIdx.x % FT;
 	const int ya = rb * ROWS + 2 * grp;
 	const Tin* src = img + blockIdx.y * img_stride + (long long)(sy + ya) * pitch + sx;
-	float2* fb = buf + grp * T;
+	float2* fb = buf + grp * FL;
 	for (int x = lane; x < T; x += FT) {
 		float a = (load_px(src, x) - mean) * mask[ya * T + x];
 		float b = (load_px(src, (long long)pitch + x) - mean) * mask[(ya + 1) * T + x];
-		fb[x] = make_float2(a, b);
+		fb[pidx(x)] = make_float2(a, b);
 	}
 	__syncthreads();
 	fft_lds<T, FT, false>(fb, tw, lane);
 	for (int k = lane; k < CX; k += FT) {
-		float2 z = fb[k];
-		float2 zc = fb[(T - k) & (T - 1)];
+		float2 z = fb[pidx(k)];
+		float2 zc = fb[pidx((T - k) & (T - 1))];
 		float2 A = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
 		float2 B = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
 		stage[k * ROWS + 2 * grp] = A;
@@ -250,7 +264,8 @@ __global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols(const float2* __res
 	constexpr int FT = T / 8;
 	constexpr int CX = T / 2 + 1;
 	constexpr int PER = T / FT;            // 8 outputs per thread
-	__shared__ float2 buf[CPW * T];
+	constexpr int FL = fft_lds_len(T);
+	__shared__ float2 buf[CPW * FL];
 	const int grp = threadIdx.x / FT, lane = threadIdx.x % FT;
 	const int kx = blockIdx.x * CPW + grp;
 	const bool live = kx < CX;
@@ -260,18 +275,18 @@ __global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols(const float2* __res
 	float acc[PER];
 #pragma unroll
 	for (int i = 0; i < PER; ++i) acc[i] = 0.f;
-	float2* fb = buf + grp * T;
+	float2* fb = buf + grp * FL;
 	const float2* src = rt + blockIdx.z * rt_stride;
 	for (int t = t0; t < t1; ++t) {
 		if (live) {
 			const float2* col = src + ((long long)t * CX + kx) * T;
-			for (int y = lane; y < T; y += FT) fb[y] = col[y];
+			for (int y = lane; y < T; y += FT) fb[pidx(y)] = col[y];
 		}
 		__syncthreads();
 		fft_lds<T, FT, false>(fb, tw, lane);
 #pragma unroll
 		for (int i = 0; i < PER; ++i) {
-			float2 c = fb[lane + i * FT];
+			float2 c = fb[pidx(lane + i * FT)];
 			acc[i] = fmaf(sqrtf(fmaf(c.x, c.x, c.y * c.y)), fac, acc[i]);
 		}
 		__syncthreads();
@@ -403,12 +418,13 @@ __global__ __launch_bounds__(2 * (T / 8)) void k_lp_x(const float* __restrict__ 
 	const float2* __restrict__ tw, float* __restrict__ q)
 {
 	constexpr int CX = T / 2 + 1, FT = T / 8, HX = T / 2;
-	__shared__ float2 buf[2 * T];
+	constexpr int FL = fft_lds_len(T);
+	__shared__ float2 buf[2 * FL];
 	if (!st[blockIdx.y].lowpassed) return;
 	const int grp = threadIdx.x / FT, lane = threadIdx.x % FT;
 	const int ya = 4 * blockIdx.x + 2 * grp;     // rows ya, ya + 1
 	const float* in = spect + (long long)blockIdx.y * T * CX;
-	float2* fb = buf + grp * T;
+	float2* fb = buf + grp * FL;
 	for (int xd = lane; xd < T; xd += FT) {
 		float v[2];
 #pragma unroll
@@ -417,20 +433,21 @@ __global__ __launch_bounds__(2 * (T / 8)) void k_lp_x(const float* __restrict__ 
 			if (xs < 0) { xs = -xs; ys = (T - yd) & (T - 1); }
 			v[r] = in[ys * CX + xs];
 		}
-		fb[xd] = make_float2(v[0], v[1]);
+		fb[pidx(xd)] = make_float2(v[0], v[1]);
 	}
 	__syncthreads();
 	fft_lds<T, FT, false>(fb, tw, lane);
 	for (int k = lane; k < T; k += FT) {
 		float f = h1[k];
-		fb[k] = make_float2(fb[k].x * f, fb[k].y * f);
+		float2 z = fb[pidx(k)];
+		fb[pidx(k)] = make_float2(z.x * f, z.y * f);
 	}
 	__syncthreads();
 	fft_lds<T, FT, true>(fb, tw, lane);
 	float* out = q + (long long)blockIdx.y * T * CX;
 	const float inv = 1.0f / (float)T;
 	for (int c = lane; c < CX; c += FT) {
-		float2 z = fb[(HX + c) & (T - 1)];
+		float2 z = fb[pidx((HX + c) & (T - 1))];
 		out[ya * CX + c] = z.x * inv;
 		out[(ya + 1) * CX + c] = z.y * inv;
 	}
@@ -443,27 +460,29 @@ __global__ __launch_bounds__(T / 8) void k_lp_y(const float* __restrict__ q,
 	const float2* __restrict__ tw, float* __restrict__ spect)
 {
 	constexpr int CX = T / 2 + 1, FT = T / 8;
-	__shared__ float2 buf[T];
+	__shared__ float2 buf[fft_lds_len(T)];
 	if (!st[blockIdx.y].lowpassed) return;
 	const int c0 = 2 * blockIdx.x;
 	const bool two = c0 + 1 < CX;
 	const float* in = q + (long long)blockIdx.y * T * CX;
 	const int lane = threadIdx.x;
 	for (int y = lane; y < T; y += FT)
-		buf[y] = make_float2(in[y * CX + c0], two ? in[y * CX + c0 + 1] : 0.f);
+		buf[pidx(y)] = make_float2(in[y * CX + c0], two ? in[y * CX + c0 + 1] : 0.f);
 	__syncthreads();
 	fft_lds<T, FT, false>(buf, tw, lane);
 	for (int k = lane; k < T; k += FT) {
 		float f = h1[k];
-		buf[k] = make_float2(buf[k].x * f, buf[k].y * f);
+		float2 z = buf[pidx(k)];
+		buf[pidx(k)] = make_float2(z.x * f, z.y * f);
 	}
 	__syncthreads();
 	fft_lds<T, FT, true>(buf, tw, lane);
 	float* out = spect + (long long)blockIdx.y * T * CX;
 	const float inv = 1.0f / (float)T;
 	for (int y = lane; y < T; y += FT) {
-		out[y * CX + c0] = buf[y].x * inv;
-		if (two) out[y * CX + c0 + 1] = buf[y].y * inv;
+		float2 z = buf[pidx(y)];
+		out[y * CX + c0] = z.x * inv;
+		if (two) out[y * CX + c0 + 1] = z.y * inv;
 	}
 }
 
