@@ -41,6 +41,27 @@ def flags(debug=False):
     return f
 
 
+HOST_SOURCES = ["host/gctf_host.cpp", "host/mrc_io.cpp", "host/gctffind_main.cpp"]
+CLI = os.path.join(HERE, "GCtfFind")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def build_cli(verbose=True):
+    """The GCtfFind command line (C++ over the C-ABI), linked to libgctf_hip.so."""
+    cxx = shutil.which("g++") or "g++"
+    srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES]
+    tmp = CLI + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-ffp-contract=off", f"-I{INCLUDE}",
+           f"-I{os.path.join(CSRC, 'host')}", "-o", tmp] + srcs + [
+        f"-L{HERE}", "-lgctf_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib",
+        "-L/opt/rocm/lib", "-lamdhip64"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, CLI)
+    return CLI
+
+
 def build(debug=False, verbose=True):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
@@ -49,6 +70,7 @@ def build(debug=False, verbose=True):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
 
 

@@ -1,0 +1,92 @@
+"""GCtfFind command line on a small tilt series against the oracle driving
+the reference's package logic (zero tilt first with a full search, then
+Refine around the best-scoring image so far; CFindSeriesCtfs.cpp:133-203)."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from mrc_util import read_mrc, write_mrc
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "gctffind_amd", "GCtfFind")
+
+
+def _gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def oracle_series(imgs, pix, tilts):
+    """The reference's tilt-series loop restated with the oracle."""
+    n = len(imgs)
+    z = int(np.argmin(np.abs(tilts)))
+    order = [z] + [i for i in range(n) if i != z]
+    res = [None] * n
+    ref = None
+    for done, i in enumerate(order):
+        nnx, nny, pitch, pixn, _ = O.rescale_sizes(imgs[i].shape[1], imgs[i].shape[0], pix)
+        fc = O.FindCtf2D(pix=pixn)
+        fc.gen_half_spectrum(O.rescale(imgs[i], pix), nnx, nny)
+
+        def full():
+            fc.set_phase(0.0, 0.0)
+            return fc.do2d()
+        if done == 0:
+            r = full()
+            ref = i
+        else:
+            rp = res[ref]
+            dm = 0.5 * (rp["df_min"] + rp["df_max"])
+            dfr = [dm, max(dm * 0.5, 5000.0 * pix * pix)]
+            mean = (rp["df_max"] + rp["df_min"]) * 0.5
+            ast = [0.0 if mean <= 0 else (rp["df_max"] - rp["df_min"]) * 0.5 / mean, 0.0]
+            r = fc.refine(np.float32(dfr), np.float32(ast), [rp["azimuth"], 0.0], [rp["ext_phase"], 0.0])
+            if res[ref]["score"] < r["score"]:
+                ref = i
+            if r["score"] <= 0:
+                r = full()
+                ref = i
+        res[i] = r
+    return res
+
+
+def test_tilt_series_cli_matches_oracle(tmp_path):
+    _gpu()
+    from gctffind_amd import synth
+    pix = 1.35
+    tilts = np.array([-6.0, -3.0, 0.0, 3.0, 6.0], np.float32)
+    imgs = []
+    for k, t in enumerate(tilts):
+        df = 30000.0 + 2000.0 * math.sin(math.radians(t))
+        imgs.append(synth.micrograph_np(1024, 1024, pix=pix, seed=300 + k, df1=df + 200.0, df2=df - 200.0,
+                                        az_deg=45.0))
+    stack = str(tmp_path / "ts.mrc")
+    write_mrc(stack, np.stack(imgs))
+    ang = tmp_path / "ts.tlt"
+    ang.write_text("".join(f"{t:.2f}\n" for t in tilts))
+    out_ctf, out_mrc = str(tmp_path / "ts_ctf.txt"), str(tmp_path / "ts_spec.mrc")
+    r = subprocess.run([CLI, "-InMrc", stack, "-OutMrc", out_mrc, "-OutCtf", out_ctf, "-AngFile", str(ang),
+                        "-PixSize", str(pix), "-kV", "300", "-Cs", "2.7", "-AmpContrast", "0.07"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rows = [ln.split() for ln in open(out_ctf).read().splitlines() if not ln.startswith("#")]
+    assert len(rows) == 5
+    ref = oracle_series(imgs, pix, tilts)
+    for i, row in enumerate(rows):
+        assert int(row[0]) == i and float(row[1]) == pytest.approx(tilts[i], abs=0.01)
+        dfmin, dfmax, az, ph, score = map(float, row[2:7])
+        assert abs(dfmin - ref[i]["df_min"]) <= 5.0, (i, row, ref[i])
+        assert abs(dfmax - ref[i]["df_max"]) <= 5.0, (i, row, ref[i])
+        assert abs(az - ref[i]["azimuth"]) <= 0.5
+        assert abs(score - ref[i]["score"]) <= 2e-4        # printed with %8.4f
+    imod = open(out_ctf.replace(".txt", "_Imod.txt")).read().splitlines()
+    assert imod[0] == "1  0  0.0  0.0  0.0  3" and len(imod) == 6
+    spec = read_mrc(out_mrc)
+    assert spec.shape == (5, 512, 512) and np.isfinite(spec).all()

@@ -165,3 +165,42 @@ def test_batch_equals_single():
         for k in ("df_min", "df_max", "azimuth", "score"):
             assert rb[i][k] == rs[k], (i, k)
             assert rb2[i][k] == rs[k], (i, k)
+
+
+def test_full_spectrum_matches_oracle():
+    img = _img(1024, 1.2, seed=9, df1=16000.0, df2=15000.0, az_deg=70.0)
+    ctx, pad, (nnx, nny, pixn) = _gpu_spectra(img, 1.2)
+    sp_g = ctx.get_half_spect(raw=False)
+    fc = O.FindCtf2D(pix=pixn)
+    fc.set_half_spect(sp_g.astype(np.float64))
+    fc.set_phase(0.0, 0.0)
+    ro = fc.do2d()
+    ctx.set_phase(0.0, 0.0)
+    rg = ctx.do2d()
+    assert rg["df_min"] == ro["df_min"] and rg["azimuth"] == ro["azimuth"]
+    fg = ctx.full_spectrum()
+    fo = fc.full_spectrum()
+    assert fg.shape == fo.shape == (512, 512)
+    scale = np.abs(fo).max()
+    assert np.abs(fg - fo).max() <= 1e-4 * scale
+    # left half carries the fitted |CTF|, right half the spectrum itself
+    assert np.array_equal(fg[:, 256:], sp_g[(np.arange(512)), :256].astype(np.float32)) or \
+        np.allclose(fg[:, 256:], fo[:, 256:], rtol=0, atol=1e-6 * scale)
+
+
+def test_gpu_radial_and_ring_tables_match_oracle():
+    # a random spectrum: any differing radial sample or ring pixel would move
+    # the 1D/2D correlations far beyond the tolerance used here
+    _dev()
+    rng = np.random.default_rng(21)
+    spect = rng.random((512, 257)).astype(np.float32)
+    ctx = _ctx()
+    ctx.setup1(300.0, 2.7, 0.07, 1.2)
+    ctx.set_half_spect(spect)
+    ctx.set_phase(0.0, 0.0)
+    fc = O.FindCtf2D(pix=1.2)
+    fc.set_half_spect(spect.astype(np.float64))
+    rg = ctx.do1d()
+    ro = fc.do1d()
+    assert abs(rg["score"] - ro["score"]) <= 1e-6
+    assert rg["df_min"] == ro["df_min"]
