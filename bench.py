@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--pix", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--mode", choices=["plain", "grid"], default="plain",
+                    help="plain: Do1D + Do2D search; grid: 256x32x36 BF16 MFMA grid search")
     ap.add_argument("--chunk", type=int, default=0, help="images per spectrum chunk (0: library default)")
     ap.add_argument("--splits", type=int, default=0, help="tile-column splits (0: library default)")
     return ap.parse_args()
@@ -72,12 +74,17 @@ def main():
     n = a.batch
     ptr = imgs.data_ptr()
 
+    grid = dict(gctf.GRID_256x32x36)
+    if a.mode == "grid":
+        run = lambda: ctx.grid_search_batch(ptr, n, a.size, a.size, a.pix, grid)
+    else:
+        run = lambda: ctx.estimate_batch(ptr, n, a.size, a.size, a.pix)
     for _ in range(a.warmup):
-        res = ctx.estimate_batch(ptr, n, a.size, a.size, a.pix)
+        res = run()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        res = ctx.estimate_batch(ptr, n, a.size, a.size, a.pix)
+        res = run()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms = dt * 1000.0 / a.steps
@@ -97,17 +104,21 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"plain CTF estimation (rescale + 144-tile spectrum + 1D/2D search), "
-                        f"batch of {n} synthetic {a.size}x{a.size} f32 micrographs at {a.pix} A",
+            "workload": (f"plain CTF estimation (rescale + 144-tile spectrum + 1D/2D search), "
+                         f"batch of {n} synthetic {a.size}x{a.size} f32 micrographs at {a.pix} A")
+            if a.mode == "plain" else
+            (f"BF16 MFMA grid search 256 df x 32 astigmatism x 36 azimuth (ring 1/30-1/4 1/A, top-16 "
+             f"FP32 re-score) after the spectrum stages, batch of {n} synthetic {a.size}x{a.size} f32 "
+             f"micrographs at {a.pix} A"),
             "batch": n, "size": a.size, "pixel_A": a.pix, "tile": 512,
-            "path": "plain",
+            "path": a.mode,
         },
     }
     dfe = [abs(0.5 * (r["df_min"] + r["df_max"]) - t[0]) for r, t in zip(res, truth)]
     out["config"]["median_abs_ddf_vs_truth_A"] = round(float(sorted(dfe)[len(dfe) // 2]), 1)
 
     if not a.no_roofline:
-        out["roofline"], out["kernels"] = roofline(ctx, ptr, n, a)
+        out["roofline"], out["kernels"] = roofline(ctx, ptr, n, a, run)
     if not a.no_cpu_baseline:
         out["cpu_baseline"], par = cpu_baseline(imgs[0].cpu().numpy(), a.pix, res[0])
         out["config"]["parity_vs_oracle_img0"] = par
@@ -149,13 +160,14 @@ def kernel_bytes(name, g, per_launch_images, nsplit=8):
     return b * c
 
 
-def roofline(ctx, ptr, n, a):
+def roofline(ctx, ptr, n, a, run):
     """One extra pass with every kernel bracketed by HIP events on the
-    library's stream; the dominant HBM-bound kernel's algorithmic bytes per
-    launch over its average launch duration."""
+    library's stream.  plain: the dominant HBM-bound kernel's algorithmic
+    bytes per launch over its average launch duration.  grid: the GEMM's
+    algorithmic flops (2 M K N) over its launch duration."""
     g = geometry(ctx, a.size, a.pix)
     ctx.profile(True)
-    ctx.estimate_batch(ptr, n, a.size, a.size, a.pix)
+    run()
     rep = ctx.profile_report()
     ctx.profile(False)
     chunk = min(a.chunk or 4, n)
@@ -170,6 +182,15 @@ def roofline(ctx, ptr, n, a):
         nb = kernel_bytes(name, g, per)
         rows[name] = dict(ms_total=round(tot, 3), launches=cnt, avg_us=round(avg_ms * 1e3, 2),
                           gbs=None if nb is None else round(nb / (avg_ms * 1e-3) / 1e9, 1))
+    if a.mode == "grid" and "k_grid_gemm" in rows:
+        gr = ctx_grid_geometry(ctx, a)
+        flops = 2.0 * gr["m"] * gr["k"] * n
+        us = rows["k_grid_gemm"]["avg_us"]
+        ach = flops / (us * 1e-6) / 1e12
+        roof = {"bound": "mfma", "achieved": round(ach, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                "frac": round(ach / 2500.0, 4), "traffic": None, "kernel": "k_grid_gemm",
+                "algorithmic_flops_per_launch": flops, "avg_launch_us": us, "dtype": "bf16"}
+        return roof, rows
     hbm = {k: v for k, v in rows.items() if v["gbs"] is not None}
     dom = max(hbm, key=lambda k: hbm[k]["ms_total"])
     per = chunk
@@ -179,6 +200,22 @@ def roofline(ctx, ptr, n, a):
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
             "algorithmic_bytes_per_launch": int(nb), "avg_launch_us": rows[dom]["avg_us"]}
     return roof, rows
+
+
+def ctx_grid_geometry(ctx, a):
+    """Candidate count and ring size of the grid search (ring [1/30, 1/4] 1/A
+    at the working pixel size, the GCC2D float membership rule)."""
+    import numpy as np
+    nnx, nny, pitch, pixn = ctx.rescale_setup(a.size, a.size, a.pix)
+    f = np.float32
+    r2 = f(512) * f(pixn)
+    lo, hi = (r2 / f(30.0)) / f(512), (r2 / f(4.0)) / f(512)
+    y, x = np.mgrid[0:512, 0:257]
+    fy = (y.astype(f) - f(256)) / f(512)
+    fx = (f(0.5) * x.astype(f)) / f(256)
+    s2 = fx * fx + fy * fy
+    k = int(((s2 >= lo * lo) & (s2 <= hi * hi)).sum())
+    return {"m": 256 * 32 * 36, "k": k}
 
 
 def cpu_baseline(img, pix, gpu_res):

@@ -22,6 +22,7 @@
 #include "gctf_internal.hpp"
 #include "k_spectrum.hpp"
 #include "k_search.hpp"
+#include "k_grid.hpp"
 
 using namespace gctf;
 
@@ -175,6 +176,7 @@ struct gctf_ctx {
 	DevBuf cbufA, cbufB, padc, bsums, rt, tpart;
 	DevBuf raw, hsum, spect, mom, q, radial, ring_spec, ring_ss, states, part2d, refine, shell_cc, fullsp,
 		moments1;
+	DevBuf g_bt, g_fn, g_cc, g_part, g_top, g_rs, g_rescored;
 	std::vector<SearchState> h_states;
 	SearchState* h_pinned = nullptr;
 	int h_pinned_n = 0;
@@ -371,10 +373,11 @@ int get_tile_tables(gctf_ctx* c, int T, TileTables** out)
 // CCTFTheory::Setup (CCTFTheory.cpp:69-86) + the kernels' amplitude phase
 // (GCalcCTF2D.cu:88-97) + resolution range (CFindCtfBase.cpp:64-68) + CC
 // frequency bounds (CFindDefocus1D.cpp:128-135, CFindDefocus2D.cpp:86-92).
-int get_search_plan(gctf_ctx* c, float kv, float cs, float ac, float pix, int T, SearchPlan** out)
+int get_search_plan(gctf_ctx* c, float kv, float cs, float ac, float pix, int T, SearchPlan** out,
+	float res_lo = 0.f, float res_hi = 0.f)
 {
-	char key[128];
-	snprintf(key, sizeof key, "%a/%a/%a/%a/%d", kv, cs, ac, pix, T);
+	char key[160];
+	snprintf(key, sizeof key, "%a/%a/%a/%a/%d/%a/%a", kv, cs, ac, pix, T, res_lo, res_hi);
 	auto it = c->splans.find(key);
 	if (it != c->splans.end()) { *out = it->second.get(); return GCTF_OK; }
 	auto sp = std::make_unique<SearchPlan>();
@@ -388,6 +391,7 @@ int get_search_plan(gctf_ctx* c, float kv, float cs, float ac, float pix, int T,
 	sp->res[0] = 25.0f * pix;
 	sp->res[1] = (2.0f * pix) / 0.8f;
 	if (sp->res[1] < 3.5f) sp->res[1] = 3.5f;
+	if (res_lo > 0.f && res_hi > 0.f) { sp->res[0] = res_lo; sp->res[1] = res_hi; }
 	const int cx = T / 2 + 1, cy = T;
 	float r1 = (float)((cx - 1) * 2) * pix;
 	sp->lo1d = r1 / sp->res[0];
@@ -422,18 +426,22 @@ int get_search_plan(gctf_ctx* c, float kv, float cs, float ac, float pix, int T,
 		}
 	const int n = (int)idx.size();
 	if (n == 0) return fail(c, GCTF_ERR_ARG, "empty correlation ring");
-	CHK(ensure_buf(c, sp->idx, sizeof(int) * n));
-	CHK(ensure_buf(c, sp->a, sizeof(float) * n));
-	CHK(ensure_buf(c, sp->b, sizeof(float) * n));
-	CHK(ensure_buf(c, sp->c2, sizeof(float) * n));
-	CHK(ensure_buf(c, sp->s2, sizeof(float) * n));
-	CHK(ensure_buf(c, sp->w, sizeof(float) * n));
-	HIPCHK(c, hipMemcpy(sp->idx.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-	HIPCHK(c, hipMemcpy(sp->a.p, a.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-	HIPCHK(c, hipMemcpy(sp->b.p, b.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-	HIPCHK(c, hipMemcpy(sp->c2.p, c2.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-	HIPCHK(c, hipMemcpy(sp->s2.p, s2.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-	HIPCHK(c, hipMemcpy(sp->w.p, w.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+	// zero-padded to a multiple of 16 (w = 0 there) for the grid GEMM's vector loads
+	const int n16 = (n + 15) / 16 * 16;
+	idx.resize(n16, 0); a.resize(n16, 0.f); b.resize(n16, 0.f); c2.resize(n16, 0.f); s2.resize(n16, 0.f);
+	w.resize(n16, 0.f);
+	CHK(ensure_buf(c, sp->idx, sizeof(int) * n16));
+	CHK(ensure_buf(c, sp->a, sizeof(float) * n16));
+	CHK(ensure_buf(c, sp->b, sizeof(float) * n16));
+	CHK(ensure_buf(c, sp->c2, sizeof(float) * n16));
+	CHK(ensure_buf(c, sp->s2, sizeof(float) * n16));
+	CHK(ensure_buf(c, sp->w, sizeof(float) * n16));
+	HIPCHK(c, hipMemcpy(sp->idx.p, idx.data(), sizeof(int) * n16, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(sp->a.p, a.data(), sizeof(float) * n16, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(sp->b.p, b.data(), sizeof(float) * n16, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(sp->c2.p, c2.data(), sizeof(float) * n16, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(sp->s2.p, s2.data(), sizeof(float) * n16, hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(sp->w.p, w.data(), sizeof(float) * n16, hipMemcpyHostToDevice));
 	sp->ring.n = n;
 	sp->ring.idx = sp->idx.as<int>();
 	sp->ring.a = sp->a.as<float>();
@@ -1217,11 +1225,23 @@ int gctf_full_spectrum(gctf_ctx* ctx, float* host_out)
 	return GCTF_OK;
 }
 
+}  // extern "C"
+
 // ---- batched path ----------------------------------------------------------
+// (the ABI functions below keep C linkage from their declarations in gctf_hip.h)
 namespace {
 
-int record_batch(gctf_ctx* c, const RescalePlan& rp, SearchPlan& sp, TileTables& tt, const void* imgs,
-	int dtype, int n, long long stride)
+// Search variants of a batch after the shared spectrum stages.
+struct BatchJob {
+	int mode = 0;                 // 0: Do1D + Do2D, 1: BF16 grid search
+	GridSpec gs{};                // mode 1
+	SearchPlan* gring = nullptr;  // mode 1: ring of the grid correlation
+	int np = 0;                   // mode 1: micrograph columns padded to 32
+	int nchunks = 0;              // mode 1: top-k row chunks
+};
+
+int record_spectra(gctf_ctx* c, const RescalePlan& rp, TileTables& tt, const void* imgs, int dtype, int n,
+	long long stride)
 {
 	const int T = c->prm.tile_size;
 	const TileGrid g = make_grid(rp.nnx, rp.nny, T);
@@ -1243,18 +1263,66 @@ int record_batch(gctf_ctx* c, const RescalePlan& rp, SearchPlan& sp, TileTables&
 			CHK(launch_spectrum<float>(c, g, tt, (const float*)src, rp.nx, stride, m, i0));
 		}
 	}
-	CHK(launch_post(c, tt, n));
+	return launch_post(c, tt, n);
+}
+
+template <int NT>
+void launch_gemm_t(gctf_ctx* c, const BatchJob& j, int k16)
+{
+	const SearchPlan& r = *j.gring;
+	PROF(c, "k_grid_gemm", k_grid_gemm<NT><<<(j.gs.m + 127) / 128, 256, 0, c->stream>>>(j.gs, r.cc, r.ring, k16,
+		c->g_bt.as<__bf16>(), c->g_fn.as<double>(), j.np, c->prm.ext_phase_deg, c->g_cc.as<float>()));
+}
+
+int launch_grid(gctf_ctx* c, const BatchJob& j, int n)
+{
+	hipStream_t s = c->stream;
+	const SearchPlan& r = *j.gring;
+	const int T = r.T, cx = T / 2 + 1;
+	const int K = r.ring.n, k16 = (K + 15) / 16 * 16;
+	PROF(c, "k_ring_gather", k_ring_gather<<<n, 1024, 0, s>>>(c->spect.as<float>(), T * cx, r.ring,
+		c->ring_spec.as<float>(), c->ring_ss.as<double>()));
+	PROF(c, "k_grid_pack", k_grid_pack<<<j.np, 256, 0, s>>>(c->ring_spec.as<float>(), K, k16, n, j.np,
+		c->g_bt.as<__bf16>(), c->g_fn.as<double>()));
+	switch (j.np / 32) {
+	case 1: launch_gemm_t<1>(c, j, k16); break;
+	case 2: launch_gemm_t<2>(c, j, k16); break;
+	case 3: launch_gemm_t<3>(c, j, k16); break;
+	case 4: launch_gemm_t<4>(c, j, k16); break;
+	case 5: launch_gemm_t<5>(c, j, k16); break;
+	case 6: launch_gemm_t<6>(c, j, k16); break;
+	case 7: launch_gemm_t<7>(c, j, k16); break;
+	default: launch_gemm_t<8>(c, j, k16); break;
+	}
+	const int rows = (j.gs.m + j.nchunks - 1) / j.nchunks;
+	PROF(c, "k_grid_topk", k_grid_topk<<<j.nchunks, 256, 0, s>>>(c->g_cc.as<float>(), j.gs.m, j.np, n, rows,
+		c->g_part.as<float2>()));
+	PROF(c, "k_grid_merge", k_grid_merge<<<n, 64, 0, s>>>(c->g_part.as<float2>(), j.nchunks, j.np,
+		c->g_top.as<int>()));
+	PROF(c, "k_grid_rescore", k_grid_rescore<<<dim3(kGridTopK / 8, kPixChunks, n), 256, 0, s>>>(j.gs, r.cc,
+		r.ring, c->ring_spec.as<float>(), c->g_top.as<int>(), c->prm.ext_phase_deg, c->g_rs.as<double2>()));
+	PROF(c, "k_grid_pick", k_grid_pick<<<(n + 63) / 64, 64, 0, s>>>(j.gs, c->g_rs.as<double2>(),
+		c->ring_ss.as<double>(), c->g_top.as<int>(), c->prm.ext_phase_deg, c->states.as<SearchState>(),
+		c->g_rescored.as<float>(), n));
+	HIPCHK(c, hipGetLastError());
+	return GCTF_OK;
+}
+
+int record_batch(gctf_ctx* c, const RescalePlan& rp, SearchPlan& sp, TileTables& tt, const void* imgs,
+	int dtype, int n, long long stride, const BatchJob& job)
+{
+	CHK(record_spectra(c, rp, tt, imgs, dtype, n, stride));
+	if (job.mode == 1) return launch_grid(c, job, n);
 	CHK(launch_1d(c, sp, tt, n, 0, 0.f, 0.f));
 	CHK(launch_2d(c, sp, n, NX_BEGIN_DO2D, std::max(c->prm.ext_phase_range_deg, 0.0f)));
 	return GCTF_OK;
 }
 
-}  // namespace
-
-int gctf_estimate_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny,
-	long long stride, float pix, gctf_result* out)
+// Shared driver of the batched entry points: plans, buffers, graph capture
+// (one executable graph per input pointer, shape and job) and read-back.
+int run_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny, long long stride,
+	float pix, gctf_result* out, BatchJob& job, const char* job_key)
 {
-	if (!ctx) return GCTF_ERR_ARG;
 	if (!dev_imgs || !out || n < 1) return fail(ctx, GCTF_ERR_ARG, "null input/output or n < 1");
 	if (dtype != GCTF_DTYPE_F32 && dtype != GCTF_DTYPE_F16) return fail(ctx, GCTF_ERR_ARG, "dtype");
 	if (stride < (long long)nx * ny) return fail(ctx, GCTF_ERR_ARG, "stride smaller than one image");
@@ -1269,26 +1337,28 @@ int gctf_estimate_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, i
 	TileTables* tt;
 	CHK(get_search_plan(ctx, ctx->prm.kv, ctx->prm.cs_mm, ctx->prm.amp_contrast, rp.pixn, T, &sp));
 	CHK(get_tile_tables(ctx, T, &tt));
+	int ring_n = sp->ring.n;
+	if (job.mode == 1) ring_n = std::max(ring_n, job.gring->ring.n);
 	const int C = std::max(1, std::min(ctx->chunk, n));
-	CHK(ensure_image_buffers(ctx, n, T, sp->ring.n));
+	CHK(ensure_image_buffers(ctx, n, T, ring_n));
 	CHK(ensure_spectrum_buffers(ctx, g, T, C));
 	if (rp.active) {
 		CHK(ensure_rescale_buffers(ctx, rp, C));
 		CHK(ensure_buf(ctx, ctx->padc, sizeof(float) * (size_t)rp.pitch * rp.nny * C));
 	}
 	if (!ctx->use_graph || ctx->prof) {
-		CHK(record_batch(ctx, rp, *sp, *tt, dev_imgs, dtype, n, stride));
+		CHK(record_batch(ctx, rp, *sp, *tt, dev_imgs, dtype, n, stride, job));
 		return read_results(ctx, n, out);
 	}
-	char key[256];
-	snprintf(key, sizeof key, "%p/%d/%d/%d/%d/%lld/%a/%d/%d", dev_imgs, dtype, n, nx, ny, stride, pix, C,
-		ctx->tile_splits);
+	char key[320];
+	snprintf(key, sizeof key, "%p/%d/%d/%d/%d/%lld/%a/%d/%d/%s", dev_imgs, dtype, n, nx, ny, stride, pix, C,
+		ctx->tile_splits, job_key);
 	GraphEntry& ge = ctx->graphs[key];
 	if (!ge.exec || ge.gen != ctx->gen) {
 		if (ge.exec) { (void)hipGraphExecDestroy(ge.exec); ge.exec = nullptr; }
 		hipGraph_t graph = nullptr;
 		HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-		int r = record_batch(ctx, rp, *sp, *tt, dev_imgs, dtype, n, stride);
+		int r = record_batch(ctx, rp, *sp, *tt, dev_imgs, dtype, n, stride, job);
 		hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
 		if (r != GCTF_OK) { if (graph) (void)hipGraphDestroy(graph); return r; }
 		HIPCHK(ctx, e);
@@ -1301,4 +1371,70 @@ int gctf_estimate_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, i
 	return read_results(ctx, n, out);
 }
 
-}  // extern "C"
+}  // namespace
+
+int gctf_estimate_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny,
+	long long stride, float pix, gctf_result* out)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	BatchJob job;
+	return run_batch(ctx, dev_imgs, dtype, n, nx, ny, stride, pix, out, job, "plain");
+}
+
+int gctf_grid_search_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny,
+	long long stride, float pix, const gctf_grid* grid, gctf_result* out, float* cc_out, int* top_out,
+	float* rescored_out)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	if (!grid || grid->n_df < 1 || grid->n_ast < 1 || grid->n_az < 1 || !(grid->res_lo > grid->res_hi)
+		|| !(grid->res_hi > 0))
+		return fail(ctx, GCTF_ERR_ARG, "bad grid");
+	if (n > 256) return fail(ctx, GCTF_ERR_UNSUPPORTED, "grid search takes at most 256 micrographs per call");
+	const long long m = (long long)grid->n_df * grid->n_ast * grid->n_az;
+	if (m > (1LL << 26)) return fail(ctx, GCTF_ERR_UNSUPPORTED, "candidate grid too large");
+	BatchJob job;
+	job.mode = 1;
+	GridSpec& g = job.gs;
+	g.n_df = grid->n_df; g.n_ast = grid->n_ast; g.n_az = grid->n_az;
+	g.df_lo = grid->df_lo;
+	g.df_step = grid->n_df > 1 ? (grid->df_hi - grid->df_lo) / (float)(grid->n_df - 1) : 0.0f;
+	g.ast_lo = grid->ast_lo;
+	g.ast_step = grid->n_ast > 1 ? (grid->ast_hi - grid->ast_lo) / (float)(grid->n_ast - 1) : 0.0f;
+	g.az_lo = grid->az_lo;
+	g.az_step = (grid->az_hi - grid->az_lo) / (float)grid->n_az;
+	g.m = (int)m;
+	job.np = (n + 31) / 32 * 32;
+	job.nchunks = 256;
+	// the ring plan of the grid correlation (resolution bounds in A)
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	RescalePlan rp;
+	CHK(make_rescale_plan(ctx, nx, ny, pix, rp));
+	CHK(get_search_plan(ctx, ctx->prm.kv, ctx->prm.cs_mm, ctx->prm.amp_contrast, rp.pixn, ctx->prm.tile_size,
+		&job.gring, grid->res_lo, grid->res_hi));
+	const int K = job.gring->ring.n, k16 = (K + 15) / 16 * 16;
+	CHK(ensure_buf(ctx, ctx->g_bt, sizeof(__bf16) * (size_t)k16 * job.np));
+	CHK(ensure_buf(ctx, ctx->g_fn, sizeof(double) * job.np));
+	CHK(ensure_buf(ctx, ctx->g_cc, sizeof(float) * (size_t)g.m * job.np));
+	CHK(ensure_buf(ctx, ctx->g_part, sizeof(float2) * (size_t)job.nchunks * job.np * kGridTopK));
+	CHK(ensure_buf(ctx, ctx->g_top, sizeof(int) * (size_t)n * kGridTopK));
+	CHK(ensure_buf(ctx, ctx->g_rs, sizeof(double2) * (size_t)n * kGridTopK * kPixChunks));
+	CHK(ensure_buf(ctx, ctx->g_rescored, sizeof(float) * (size_t)n * kGridTopK));
+	char key[160];
+	snprintf(key, sizeof key, "grid/%d/%d/%d/%a/%a/%a/%a/%a/%a/%a/%a", g.n_df, g.n_ast, g.n_az, g.df_lo,
+		g.df_step, g.ast_lo, g.ast_step, g.az_lo, g.az_step, grid->res_lo, grid->res_hi);
+	CHK(run_batch(ctx, dev_imgs, dtype, n, nx, ny, stride, pix, out, job, key));
+	if (cc_out) {
+		std::vector<float> all((size_t)g.m * job.np);
+		HIPCHK(ctx, hipMemcpy(all.data(), ctx->g_cc.p, sizeof(float) * all.size(), hipMemcpyDeviceToHost));
+		for (int i = 0; i < n; ++i)
+			for (int mm = 0; mm < g.m; ++mm) cc_out[(size_t)i * g.m + mm] = all[(size_t)mm * job.np + i];
+	}
+	if (top_out)
+		HIPCHK(ctx, hipMemcpy(top_out, ctx->g_top.p, sizeof(int) * n * kGridTopK, hipMemcpyDeviceToHost));
+	if (rescored_out)
+		HIPCHK(ctx, hipMemcpy(rescored_out, ctx->g_rescored.p, sizeof(float) * n * kGridTopK,
+			hipMemcpyDeviceToHost));
+	return GCTF_OK;
+}
+
+

@@ -26,7 +26,7 @@ EXPORTS = [
     "gctf_set_phase", "gctf_gen_half_spectrum", "gctf_do1d", "gctf_refine1d", "gctf_do2d",
     "gctf_refine", "gctf_spect_size", "gctf_get_half_spect", "gctf_set_half_spect",
     "gctf_full_spectrum", "gctf_estimate_batch", "gctf_tune", "gctf_profile",
-    "gctf_profile_report",
+    "gctf_profile_report", "gctf_grid_search_batch",
 ]
 
 
@@ -45,6 +45,19 @@ class GctfResult(ctypes.Structure):
     def as_dict(self):
         return dict(df_min=self.df_min, df_max=self.df_max, azimuth=self.azimuth_deg,
                     ext_phase=self.ext_phase_deg, score=self.score, ctf_res=self.ctf_res)
+
+
+class GctfGrid(ctypes.Structure):
+    _fields_ = [("df_lo", ctypes.c_float), ("df_hi", ctypes.c_float), ("n_df", ctypes.c_int),
+                ("ast_lo", ctypes.c_float), ("ast_hi", ctypes.c_float), ("n_ast", ctypes.c_int),
+                ("az_lo", ctypes.c_float), ("az_hi", ctypes.c_float), ("n_az", ctypes.c_int),
+                ("res_lo", ctypes.c_float), ("res_hi", ctypes.c_float)]
+
+
+# the candidate grid of the 256-micrograph BF16 search (256 df x 32 ast x 36 az,
+# ring 1/30 - 1/4 1/A)
+GRID_256x32x36 = dict(df_lo=3000.0, df_hi=43000.0, n_df=256, ast_lo=0.0, ast_hi=0.06, n_ast=32,
+                      az_lo=-90.0, az_hi=90.0, n_az=36, res_lo=30.0, res_hi=4.0)
 
 
 class GctfError(RuntimeError):
@@ -90,6 +103,8 @@ def load():
         "gctf_tune": (ci, [vp, ci, ci, ci]),
         "gctf_profile": (ci, [vp, ci]),
         "gctf_profile_report": (ci, [vp, ci, fp, fp, fp, fp]),
+        "gctf_grid_search_batch": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf,
+                                        ctypes.POINTER(GctfGrid), fp, fp, fp, fp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -213,6 +228,22 @@ class Context:
         self.check(self.L.gctf_estimate_batch(self.h, ctypes.c_void_p(dev_ptr), dtype, n, nx, ny,
                                               stride, pix, ctypes.cast(res, ctypes.c_void_p)))
         return [r.as_dict() for r in res]
+
+    def grid_search_batch(self, dev_ptr, n, nx, ny, pix, grid, dtype=DTYPE_F32, stride=None,
+                          debug=False):
+        stride = nx * ny if stride is None else stride
+        g = GctfGrid(**grid)
+        m = g.n_df * g.n_ast * g.n_az
+        res = (GctfResult * n)()
+        cc = np.zeros((n, m), np.float32) if debug else None
+        top = np.zeros((n, 16), np.int32) if debug else None
+        rs = np.zeros((n, 16), np.float32) if debug else None
+        p = (lambda a: _np_ptr(a) if a is not None else None)
+        self.check(self.L.gctf_grid_search_batch(self.h, ctypes.c_void_p(dev_ptr), dtype, n, nx, ny, stride,
+                                                 pix, ctypes.byref(g), ctypes.cast(res, ctypes.c_void_p),
+                                                 p(cc), p(top), p(rs)))
+        out = [r.as_dict() for r in res]
+        return (out, cc, top, rs) if debug else out
 
     def tune(self, chunk=0, tile_splits=0, use_graph=0):
         self.check(self.L.gctf_tune(self.h, chunk, tile_splits, use_graph))

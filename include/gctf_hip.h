@@ -119,6 +119,32 @@ GCTF_API int gctf_full_spectrum(gctf_ctx* ctx, float* host_out);
 GCTF_API int gctf_estimate_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx,
 	int ny, long long stride, float pix, gctf_result* out);
 
+/* Exhaustive grid search on MFMA (no reference counterpart; it replaces
+ * the coordinate descent of CFindDefocus2D::mIterate with a full grid).
+ * Candidate m = (i_df * n_ast + i_ast) * n_az + i_az with mean defocus
+ * df_lo + i_df (df_hi - df_lo)/(n_df - 1) A, astigmatism ratio
+ * ast_lo + i_ast (ast_hi - ast_lo)/(n_ast - 1), azimuth
+ * az_lo + i_az (az_hi - az_lo)/n_az degrees.  The correlation ring is
+ * [1/res_lo, 1/res_hi] 1/A with the GCC2D weighting (CFindDefocus2D.cpp:91). */
+typedef struct {
+	float df_lo, df_hi;
+	int n_df;
+	float ast_lo, ast_hi;
+	int n_ast;
+	float az_lo, az_hi;
+	int n_az;
+	float res_lo, res_hi;     /* A, res_lo > res_hi */
+} gctf_grid;
+
+/* Spectrum stages as gctf_estimate_batch, then CC of every grid candidate
+ * against every micrograph as a BF16 GEMM with FP32 accumulation, the best
+ * 16 per micrograph re-scored in FP32 (FP64 sums); out gets the best
+ * re-scored candidate.  n <= 256.  Optional host outputs: cc_out [n][M]
+ * BF16-GEMM CCs, top_out [n][16] candidate indices, rescored_out [n][16]. */
+GCTF_API int gctf_grid_search_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx,
+	int ny, long long stride, float pix, const gctf_grid* grid, gctf_result* out, float* cc_out,
+	int* top_out, float* rescored_out);
+
 /* Knobs of the batched path (0 keeps the current value): images per
  * spectrum chunk, tile-column splits, graph use (1 on, -1 off). */
 GCTF_API int gctf_tune(gctf_ctx* ctx, int chunk, int tile_splits, int use_graph);

@@ -74,6 +74,8 @@ def lib():
         "orc_fc_refine": (None, [_c_void, _f32p, _f32p, _f32p, _f32p]),
         "orc_fc_results": (None, [_c_void, _f32p]),
         "orc_fc_full_spectrum": (None, [_c_void, _f64p]),
+        "orc_grid_cc": (None, [_f64p, _c_int, _c_int, _c_float, _c_float, _c_float, _c_float, _c_float,
+                               _f32p, _i32p, _c_float, _c_float, _f64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -317,3 +319,17 @@ def estimate(img, pix, kv=300.0, cs=2.7, ac=0.07, tile=512, log_spect=0, phase=(
     fc.set_phase(phase[0], max(phase[1], 0.0))
     res = fc.do2d()
     return res, fc
+
+
+def grid_cc(spect, grid, pix, kv=300.0, cs=2.7, ac=0.07, ext_phase_deg=0.0):
+    """FP64 CC of every grid candidate against one half spectrum.  grid: dict
+    with df_lo, df_hi, n_df, ast_lo, ast_hi, n_ast, az_lo, az_hi, n_az,
+    res_lo, res_hi (the gctf_grid fields)."""
+    spect = np.ascontiguousarray(spect, np.float64)
+    gf = np.array([grid["df_lo"], grid["df_hi"], grid["ast_lo"], grid["ast_hi"], grid["az_lo"],
+                   grid["az_hi"]], np.float32)
+    gi = np.array([grid["n_df"], grid["n_ast"], grid["n_az"]], np.int32)
+    out = np.zeros(int(gi.prod()), np.float64)
+    lib().orc_grid_cc(spect, spect.shape[1], spect.shape[0], kv, cs, ac, pix, ext_phase_deg, gf, gi,
+                      grid["res_lo"], grid["res_hi"], out)
+    return out

@@ -1286,3 +1286,71 @@ ORC_EXPORT void orc_fc_full_spectrum(orc_fc* fc, double* out)
 		}
 	free(ctf);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Grid correlation (the exhaustive counterpart of the 2D search): FP64 CC  */
+/* of every candidate m = (i_df * n_ast + i_ast) * n_az + i_az against one  */
+/* spectrum, with the GCalcCTF2D / GCC2D formulas (GCalcCTF2D.cu:15-38,      */
+/* GCC2D.cu:13-61) on the ring [1/res_lo, 1/res_hi] 1/A, B = 16.            */
+/* Candidate values are IEEE single: df = df_lo + i*df_step etc., with      */
+/* df_step = (df_hi - df_lo)/(n_df - 1), ast likewise, az_step =            */
+/* (az_hi - az_lo)/n_az, and the CTF parameters as CFindDefocus2D::         */
+/* mCorrelate forms them (CFindDefocus2D.cpp:363-380).                      */
+/* ------------------------------------------------------------------------ */
+ORC_EXPORT void orc_grid_cc(const double* spect, int cmpx, int cmpy, float kv, float cs, float ac,
+	float pix, float ext_phase_deg, const float* gf, const int* gi, float res_lo, float res_hi,
+	double* cc_out)
+{
+	orc_ctf p;
+	ctf_setup(&p, kv, cs, ac, pix, 0.0f);
+	const float df_lo = gf[0], df_hi = gf[1], ast_lo = gf[2], ast_hi = gf[3], az_lo = gf[4], az_hi = gf[5];
+	const int n_df = gi[0], n_ast = gi[1], n_az = gi[2];
+	const float df_step = n_df > 1 ? (df_hi - df_lo) / (float)(n_df - 1) : 0.0f;
+	const float ast_step = n_ast > 1 ? (ast_hi - ast_lo) / (float)(n_ast - 1) : 0.0f;
+	const float az_step = (az_hi - az_lo) / (float)n_az;
+	float r2 = (float)cmpy * pix;
+	float flo = r2 / res_lo, fhi = r2 / res_hi, lo2, hi2, s2f;
+	ring_bounds(flo, fhi, cmpy, &lo2, &hi2);
+	int nr = orc_ring_pixels(cmpx, cmpy, flo, fhi, NULL);
+	double* rs2 = (double*)malloc(sizeof(double) * nr);
+	double* rth = (double*)malloc(sizeof(double) * nr);
+	double* rw = (double*)malloc(sizeof(double) * nr);
+	double* rsp = (double*)malloc(sizeof(double) * nr);
+	double ss = 0.0;
+	int k = 0;
+	for (int y = 0; y < cmpy; y++)
+		for (int x = 0; x < cmpx; x++) {
+			if (!in_ring(x, y, cmpx, cmpy, lo2, hi2, &s2f)) continue;
+			double fx = (double)((float)x * 0.5f / (float)(cmpx - 1));
+			double fy = (double)((float)(y - cmpy / 2) / (float)cmpy);
+			rs2[k] = fx * fx + fy * fy;
+			rth[k] = atan(fy / (fx + (double)1e-30f));
+			rw[k] = exp(-16.0 * (double)s2f);
+			rsp[k] = spect[(size_t)y * cmpx + x];
+			ss += rsp[k] * rsp[k];
+			k++;
+		}
+	const float d2r = 0.01745329f;
+	const double wl = p.wl, csp = p.cs, pi2 = (double)3.1415926f;
+	const int m = n_df * n_ast * n_az;
+	for (int mm = 0; mm < m; mm++) {
+		int iaz = mm % n_az, rest = mm / n_az, iast = rest % n_ast, idf = rest / n_ast;
+		float dfm = df_lo + (float)idf * df_step;
+		float ast = ast_lo + (float)iast * ast_step;
+		float az = az_lo + (float)iaz * az_step;
+		float ph_rad = ext_phase_deg * d2r, azr = az * d2r;
+		float dmin = dfm * (1.0f - ast) / p.pix, dmax = dfm * (1.0f + ast) / p.pix;
+		float mean = 0.5f * (dmin + dmax), sig = 0.5f * (dmax - dmin);
+		float add = p.amp_phase + ph_rad;
+		double s = 0.0, s1 = 0.0;
+		for (int i = 0; i < nr; i++) {
+			double df = (double)mean + (double)sig * cos(2.0 * (rth[i] - (double)azr));
+			double c = -sin((double)add + pi2 * wl * rs2[i] * (df - 0.5 * rs2[i] * wl * wl * csp));
+			c = (fabs(c) - 0.5) * rw[i];
+			s += c * rsp[i]; s1 += c * c;
+		}
+		double d = sqrt(s1 * ss);
+		cc_out[mm] = d == 0 ? 0.0 : s / d;
+	}
+	free(rs2); free(rth); free(rw); free(rsp);
+}
