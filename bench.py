@@ -39,7 +39,22 @@ def parse():
                     help="plain: Do1D + Do2D search; grid: 256x32x36 BF16 MFMA grid search")
     ap.add_argument("--chunk", type=int, default=0, help="images per spectrum chunk (0: library default)")
     ap.add_argument("--splits", type=int, default=0, help="tile-column splits (0: library default)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB, --steps times (the known-size "
+                         "kernel that scales FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def copy_calibration(torch, mib, steps):
+    """A plain copy of known size: each launch reads and writes mib MiB."""
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": steps}),
+          flush=True)
 
 
 def make_batch(torch, n, size, pix, seed0=1000):
@@ -66,6 +81,9 @@ def main():
               f"(--gpus {a.gpus} requested)", file=sys.stderr)
         sys.exit(2)
     import torch
+    if a.copy_calibration:
+        copy_calibration(torch, a.copy_calibration, a.steps)
+        return
     from gctffind_amd import gctf
 
     ctx = gctf.Context()

@@ -176,7 +176,7 @@ struct gctf_ctx {
 	DevBuf cbufA, cbufB, padc, bsums, rt, tpart;
 	DevBuf raw, hsum, spect, mom, q, radial, ring_spec, ring_ss, states, part2d, refine, shell_cc, fullsp,
 		moments1;
-	DevBuf g_bt, g_fn, g_cc, g_part, g_top, g_rs, g_rescored;
+	DevBuf g_bt, g_fn, g_gt, g_cc, g_part, g_top, g_rs, g_rescored;
 	std::vector<SearchState> h_states;
 	SearchState* h_pinned = nullptr;
 	int h_pinned_n = 0;
@@ -1267,11 +1267,13 @@ int record_spectra(gctf_ctx* c, const RescalePlan& rp, TileTables& tt, const voi
 }
 
 template <int NT>
-void launch_gemm_t(gctf_ctx* c, const BatchJob& j, int k16)
+void launch_gemm_t(gctf_ctx* c, const BatchJob& j, int kpad)
 {
 	const SearchPlan& r = *j.gring;
-	PROF(c, "k_grid_gemm", k_grid_gemm<NT><<<(j.gs.m + 127) / 128, 256, 0, c->stream>>>(j.gs, r.cc, r.ring, k16,
-		c->g_bt.as<__bf16>(), c->g_fn.as<double>(), j.np, c->prm.ext_phase_deg, c->g_cc.as<float>()));
+	const int per_wg = 32 * kGemmWaves;
+	PROF(c, "k_grid_gemm", k_grid_gemm<NT><<<(j.gs.m + per_wg - 1) / per_wg, 64 * kGemmWaves, 0, c->stream>>>(
+		j.gs, r.cc, c->g_gt.as<float>(), kpad / kGemmKs, c->g_bt.as<__bf16>(), c->g_fn.as<double>(),
+		c->prm.ext_phase_deg, c->g_cc.as<float>()));
 }
 
 int launch_grid(gctf_ctx* c, const BatchJob& j, int n)
@@ -1279,20 +1281,22 @@ int launch_grid(gctf_ctx* c, const BatchJob& j, int n)
 	hipStream_t s = c->stream;
 	const SearchPlan& r = *j.gring;
 	const int T = r.T, cx = T / 2 + 1;
-	const int K = r.ring.n, k16 = (K + 15) / 16 * 16;
+	const int K = r.ring.n, kpad = (K + kGemmKs - 1) / kGemmKs * kGemmKs;
 	PROF(c, "k_ring_gather", k_ring_gather<<<n, 1024, 0, s>>>(c->spect.as<float>(), T * cx, r.ring,
 		c->ring_spec.as<float>(), c->ring_ss.as<double>()));
-	PROF(c, "k_grid_pack", k_grid_pack<<<j.np, 256, 0, s>>>(c->ring_spec.as<float>(), K, k16, n, j.np,
+	PROF(c, "k_grid_tables", k_grid_tables<<<(kpad + 255) / 256, 256, 0, s>>>(r.cc, r.ring,
+		c->prm.ext_phase_deg, kpad, c->g_gt.as<float>()));
+	PROF(c, "k_grid_pack", k_grid_pack<<<j.np, 256, 0, s>>>(c->ring_spec.as<float>(), K, kpad, n, j.np,
 		c->g_bt.as<__bf16>(), c->g_fn.as<double>()));
 	switch (j.np / 32) {
-	case 1: launch_gemm_t<1>(c, j, k16); break;
-	case 2: launch_gemm_t<2>(c, j, k16); break;
-	case 3: launch_gemm_t<3>(c, j, k16); break;
-	case 4: launch_gemm_t<4>(c, j, k16); break;
-	case 5: launch_gemm_t<5>(c, j, k16); break;
-	case 6: launch_gemm_t<6>(c, j, k16); break;
-	case 7: launch_gemm_t<7>(c, j, k16); break;
-	default: launch_gemm_t<8>(c, j, k16); break;
+	case 1: launch_gemm_t<1>(c, j, kpad); break;
+	case 2: launch_gemm_t<2>(c, j, kpad); break;
+	case 3: launch_gemm_t<3>(c, j, kpad); break;
+	case 4: launch_gemm_t<4>(c, j, kpad); break;
+	case 5: launch_gemm_t<5>(c, j, kpad); break;
+	case 6: launch_gemm_t<6>(c, j, kpad); break;
+	case 7: launch_gemm_t<7>(c, j, kpad); break;
+	default: launch_gemm_t<8>(c, j, kpad); break;
 	}
 	const int rows = (j.gs.m + j.nchunks - 1) / j.nchunks;
 	PROF(c, "k_grid_topk", k_grid_topk<<<j.nchunks, 256, 0, s>>>(c->g_cc.as<float>(), j.gs.m, j.np, n, rows,
@@ -1411,8 +1415,9 @@ int gctf_grid_search_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n
 	CHK(make_rescale_plan(ctx, nx, ny, pix, rp));
 	CHK(get_search_plan(ctx, ctx->prm.kv, ctx->prm.cs_mm, ctx->prm.amp_contrast, rp.pixn, ctx->prm.tile_size,
 		&job.gring, grid->res_lo, grid->res_hi));
-	const int K = job.gring->ring.n, k16 = (K + 15) / 16 * 16;
-	CHK(ensure_buf(ctx, ctx->g_bt, sizeof(__bf16) * (size_t)k16 * job.np));
+	const int K = job.gring->ring.n, kpad = (K + kGemmKs - 1) / kGemmKs * kGemmKs;
+	CHK(ensure_buf(ctx, ctx->g_bt, sizeof(__bf16) * (size_t)kpad * job.np));
+	CHK(ensure_buf(ctx, ctx->g_gt, sizeof(float) * (size_t)kpad * kGemmTabs));
 	CHK(ensure_buf(ctx, ctx->g_fn, sizeof(double) * job.np));
 	CHK(ensure_buf(ctx, ctx->g_cc, sizeof(float) * (size_t)g.m * job.np));
 	CHK(ensure_buf(ctx, ctx->g_part, sizeof(float2) * (size_t)job.nchunks * job.np * kGridTopK));

@@ -45,16 +45,21 @@ __device__ __forceinline__ void grid_candidate(const GridSpec& g, int m, float& 
 	az = g.az_lo + (float)iaz * g.az_step;
 }
 
+constexpr int kGemmKs = 32;     // ring pixels per pipeline stage of the GEMM
+constexpr int kGemmTabs = 8;    // per-pixel tables of a stage (6 used)
+constexpr int kGemmWaves = 8;   // waves per workgroup, 32 candidates each
+
 // Ring spectra of the batch in the B-operand layout:
-// bt[(k / 16) * np + n][k % 16] as bf16, zero for k >= K or n >= nimg;
-// fn[n] = sum over k of the bf16 values squared (FP64).
-__global__ __launch_bounds__(256) void k_grid_pack(const float* __restrict__ ring_spec, int K, int k16,
+// bt[(k / 16) * np + n][k % 16] as bf16, zero for k >= K or n >= nimg
+// (kpad = K rounded up to kGemmKs); fn[n] = sum over k of the bf16 values
+// squared (FP64).
+__global__ __launch_bounds__(256) void k_grid_pack(const float* __restrict__ ring_spec, int K, int kpad,
 	int nimg, int np, __bf16* __restrict__ bt, double* __restrict__ fn)
 {
 	__shared__ double red[256];
 	const int n = blockIdx.x;
 	double acc = 0.0;
-	for (int k = threadIdx.x; k < k16; k += 256) {
+	for (int k = threadIdx.x; k < kpad; k += 256) {
 		float v = (n < nimg && k < K) ? ring_spec[(long long)n * K + k] : 0.f;
 		__bf16 h = (__bf16)v;
 		float r = (float)h;
@@ -70,72 +75,141 @@ __global__ __launch_bounds__(256) void k_grid_pack(const float* __restrict__ rin
 	if (threadIdx.x == 0) fn[n] = red[0];
 }
 
-// The GEMM.  One workgroup = 4 waves = 128 candidates x (32 * NT)
-// micrographs; grid (ceil(M / 128)).  cc: [M][np] floats (m-major).
+// Per-pixel tables of the GEMM's CTF generation, in revolutions of the
+// hardware sine (|sin(pi u)| = |sin(2 pi (u / 2))|):
+//   u / 2 = ga mean + gac (sig cos 2az) + gas (sig sin 2az) + gnb
+// with ga = a / 2, gac = a cos(2 theta) / 2, gas = a sin(2 theta) / 2,
+// gnb = (ph0 - b) / 2 (the grid shares one phase shift ph0), and
+// c = |sin| w - w / 2.  Layout gt[stage][table][32]; zero past K.
+__global__ __launch_bounds__(256) void k_grid_tables(CtfConsts cc, RingTables ring, float ext_phase_deg,
+	int kpad, float* __restrict__ gt)
+{
+	const int k = blockIdx.x * 256 + threadIdx.x;
+	if (k >= kpad) return;
+	const float ph0 = cand_ctf(cc, 0.f, 0.f, 0.f, ext_phase_deg).ph0;
+	float t[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+	if (k < ring.n) {
+		const float a = ring.a[k];
+		t[0] = 0.5f * a;
+		t[1] = 0.5f * a * ring.c2[k];
+		t[2] = 0.5f * a * ring.s2[k];
+		t[3] = 0.5f * (ph0 - ring.b[k]);
+		t[4] = ring.w[k];
+		t[5] = 0.5f * ring.w[k];
+	}
+	float* o = gt + (long long)(k / kGemmKs) * kGemmTabs * kGemmKs + (k % kGemmKs);
+#pragma unroll
+	for (int i = 0; i < 6; ++i) o[i * kGemmKs] = t[i];
+	o[6 * kGemmKs] = 0.f;
+	o[7 * kGemmKs] = 0.f;
+}
+
+// The GEMM.  One workgroup = 8 waves = 256 candidates x NP = 32 NT
+// micrograph columns; grid (ceil(M / 256)).  Every wave holds its
+// 32 x NP accumulator tile for the whole K loop; the B tile of a stage
+// (32 ring pixels x NP columns, bf16) and the stage's pixel tables are
+// staged once per workgroup in LDS, double-buffered: the global loads of
+// stage s + 1 are in flight while stage s is computed.
+// cc: [M][NP] floats (m-major), CC = numerator / sqrt(E_m F_n).
 template <int NT>
-__global__ __launch_bounds__(256) void k_grid_gemm(GridSpec g, CtfConsts cc, RingTables ring, int k16,
-	const __bf16* __restrict__ bt, const double* __restrict__ fn, int np, float ext_phase_deg,
+__global__ __launch_bounds__(512) void k_grid_gemm(GridSpec g, CtfConsts cc, const float* __restrict__ gt,
+	int nst, const __bf16* __restrict__ bt, const double* __restrict__ fn, float ext_phase_deg,
 	float* __restrict__ out)
 {
-	__shared__ float s_e[4][32];
-	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	constexpr int NP = 32 * NT;
+	constexpr int BV = NP * kGemmKs * 2 / 16;          // 16-byte vectors per B stage
+	constexpr int BPT = (BV + 511) / 512;              // per thread
+	constexpr int TV = kGemmTabs * kGemmKs / 4;        // float4 per table stage (64)
+	__shared__ uint4 s_b[2][BV];
+	__shared__ float4 s_t[2][TV];
+	__shared__ float s_e[kGemmWaves][32];
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int half = lane >> 5, row = lane & 31;
-	const int m = blockIdx.x * 128 + wave * 32 + row;
-	const bool live = m < g.m;
-	// this lane's candidate (GCalcCTF2D::DoIt form, as in the 2D search)
-	float mean = 0.f, sig = 0.f, ca = 1.f, sa = 0.f, ph0 = 0.f;
-	if (live) {
+	const int m = blockIdx.x * (32 * kGemmWaves) + wave * 32 + row;
+	float h_mean = 0.f, h_sc = 0.f, h_ss = 0.f;
+	if (m < g.m) {
 		float dfm, ast, az;
 		grid_candidate(g, m, dfm, ast, az);
-		CandCtf c = cand_ctf(cc, dfm, ast, az, ext_phase_deg);
-		mean = c.mean; sig = c.sig; ca = c.ca; sa = c.sa; ph0 = c.ph0;
+		const CandCtf c = cand_ctf(cc, dfm, ast, az, ext_phase_deg);
+		h_mean = c.mean;
+		h_sc = c.sig * c.ca;
+		h_ss = c.sig * c.sa;
 	}
-	floatx16 acc[NT];
+	const uint4* gb = reinterpret_cast<const uint4*>(bt);
+	const float4* gtab = reinterpret_cast<const float4*>(gt);
+	// this thread's share of a B stage: vectors v0 and (BV > 512 only) v1
+	static_assert(BV <= 1024, "at most two B vectors per thread");
+	const int v0 = tid < BV ? tid : BV - 1;
+	const int v1 = tid + 512 < BV ? tid + 512 : BV - 1;
+	uint4 pb0 = gb[v0], pb1 = BV > 512 ? gb[v1] : pb0;
+	float4 pt = gtab[tid < TV ? tid : 0];
+	if (tid < BV) s_b[0][tid] = pb0;
+	if (BV > 512 && tid + 512 < BV) s_b[0][tid + 512] = pb1;
+	if (tid < TV) s_t[0][tid] = pt;
+	__syncthreads();
+	floatx16 acc[NT], accE;
 #pragma unroll
 	for (int t = 0; t < NT; ++t)
 #pragma unroll
 		for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
-	float e = 0.f;
-	const int steps = k16 >> 4;
-	for (int st = 0; st < steps; ++st) {
-		const int k0 = st * 16 + half * 8;   // ring tables are zero-padded to k16
-		float ra[8], rb[8], rc[8], rs[8], rw[8];
-		*reinterpret_cast<float4*>(ra) = *reinterpret_cast<const float4*>(ring.a + k0);
-		*reinterpret_cast<float4*>(ra + 4) = *reinterpret_cast<const float4*>(ring.a + k0 + 4);
-		*reinterpret_cast<float4*>(rb) = *reinterpret_cast<const float4*>(ring.b + k0);
-		*reinterpret_cast<float4*>(rb + 4) = *reinterpret_cast<const float4*>(ring.b + k0 + 4);
-		*reinterpret_cast<float4*>(rc) = *reinterpret_cast<const float4*>(ring.c2 + k0);
-		*reinterpret_cast<float4*>(rc + 4) = *reinterpret_cast<const float4*>(ring.c2 + k0 + 4);
-		*reinterpret_cast<float4*>(rs) = *reinterpret_cast<const float4*>(ring.s2 + k0);
-		*reinterpret_cast<float4*>(rs + 4) = *reinterpret_cast<const float4*>(ring.s2 + k0 + 4);
-		*reinterpret_cast<float4*>(rw) = *reinterpret_cast<const float4*>(ring.w + k0);
-		*reinterpret_cast<float4*>(rw + 4) = *reinterpret_cast<const float4*>(ring.w + k0 + 4);
-		bf16x8 a;
 #pragma unroll
-		for (int j = 0; j < 8; ++j) {
-			const float gg = fmaf(rc[j], ca, rs[j] * sa);
-			const float df = fmaf(sig, gg, mean);
-			const float u = fmaf(ra[j], df, ph0 - rb[j]);
-			// |sin(pi u)| = |sin(2 pi (u / 2))|; the hardware sine takes revolutions
-			const float sv = fabsf(__builtin_amdgcn_sinf(0.5f * u));
-			const float cv = live ? (sv - 0.5f) * rw[j] : 0.f;
-			const __bf16 h = (__bf16)cv;
-			const float hr = (float)h;
-			e = fmaf(hr, hr, e);
-			a[j] = h;
+	for (int r = 0; r < 16; ++r) accE[r] = 0.f;
+	for (int st = 0; st < nst; ++st) {
+		const int cur = st & 1;
+		const bool more = st + 1 < nst;
+		if (more) {
+			const uint4* src = gb + (long long)(st + 1) * BV;
+			pb0 = src[v0];
+			if (BV > 512) pb1 = src[v1];
+			pt = gtab[(long long)(st + 1) * TV + (tid < TV ? tid : 0)];
 		}
-		const __bf16* bp = bt + ((long long)st * np + row) * 16 + half * 8;
+		const float* tb = reinterpret_cast<const float*>(s_t[cur]);
 #pragma unroll
-		for (int t = 0; t < NT; ++t) {
-			const bf16x8 b = *reinterpret_cast<const bf16x8*>(bp + (long long)t * 32 * 16);
-			acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
+		for (int sl = 0; sl < 2; ++sl) {
+			const int k0 = sl * 16 + half * 8;
+			float ga[8], gac[8], gas[8], gnb[8], w[8], hw[8];
+#pragma unroll
+			for (int q = 0; q < 2; ++q) {
+				*reinterpret_cast<float4*>(ga + 4 * q) = *reinterpret_cast<const float4*>(tb + 0 * kGemmKs + k0 + 4 * q);
+				*reinterpret_cast<float4*>(gac + 4 * q) = *reinterpret_cast<const float4*>(tb + 1 * kGemmKs + k0 + 4 * q);
+				*reinterpret_cast<float4*>(gas + 4 * q) = *reinterpret_cast<const float4*>(tb + 2 * kGemmKs + k0 + 4 * q);
+				*reinterpret_cast<float4*>(gnb + 4 * q) = *reinterpret_cast<const float4*>(tb + 3 * kGemmKs + k0 + 4 * q);
+				*reinterpret_cast<float4*>(w + 4 * q) = *reinterpret_cast<const float4*>(tb + 4 * kGemmKs + k0 + 4 * q);
+				*reinterpret_cast<float4*>(hw + 4 * q) = *reinterpret_cast<const float4*>(tb + 5 * kGemmKs + k0 + 4 * q);
+			}
+			bf16x8 a;
+#pragma unroll
+			for (int j = 0; j < 8; ++j) {
+				const float u = fmaf(gac[j], h_sc, fmaf(gas[j], h_ss, fmaf(ga[j], h_mean, gnb[j])));
+				a[j] = (__bf16)fmaf(fabsf(__builtin_amdgcn_sinf(u)), w[j], -hw[j]);
+			}
+			const char* bb = reinterpret_cast<const char*>(s_b[cur]) + sl * NP * 32 + row * 32 + half * 16;
+#pragma unroll
+			for (int t = 0; t < NT; ++t) {
+				const bf16x8 b = *reinterpret_cast<const bf16x8*>(bb + t * 32 * 32);
+				acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
+			}
+			// the A fragment is also the B fragment of A^T: diag(A A^T) = E_m
+			accE = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, a, accE, 0, 0, 0);
 		}
+		if (more) {
+			if (tid < BV) s_b[cur ^ 1][tid] = pb0;
+			if (BV > 512 && tid + 512 < BV) s_b[cur ^ 1][tid + 512] = pb1;
+			if (tid < TV) s_t[cur ^ 1][tid] = pt;
+		}
+		__syncthreads();
 	}
-	// E_m: the two k-halves of a row live in lanes l and l ^ 32
-	e += __shfl_xor(e, 32);
-	if (half == 0) s_e[wave][row] = e;
+	// E_m = D[c][c] of A A^T: row 8 (r / 4) + 4 h + r % 4 = c holds in lane
+	// c + 32 h with h = (c % 8) / 4, register r = 4 (c / 8) + c % 4
+	if (half == ((row & 7) >> 2)) {
+		const int rr = 4 * (row >> 3) + (row & 3);
+		float ev = 0.f;
+#pragma unroll
+		for (int r = 0; r < 16; ++r) ev = r == rr ? accE[r] : ev;
+		s_e[wave][row] = ev;
+	}
 	__syncthreads();
-	const int mrow0 = blockIdx.x * 128 + wave * 32;
+	const int mrow0 = blockIdx.x * (32 * kGemmWaves) + wave * 32;
 #pragma unroll
 	for (int t = 0; t < NT; ++t) {
 		const int col = t * 32 + row;
@@ -146,7 +220,7 @@ __global__ __launch_bounds__(256) void k_grid_gemm(GridSpec g, CtfConsts cc, Rin
 			const int mm = mrow0 + drow;
 			if (mm >= g.m) continue;
 			const double d = sqrt((double)s_e[wave][drow] * f);
-			out[(long long)mm * np + col] = d > 0.0 ? (float)((double)acc[t][r] / d) : 0.f;
+			out[(long long)mm * NP + col] = d > 0.0 ? (float)((double)acc[t][r] / d) : 0.f;
 		}
 	}
 }
