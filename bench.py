@@ -1,15 +1,22 @@
 """Throughput benchmark of the CTF-estimation path on one MI355X.
 
 Workload: a batch of synthetic 4096 x 4096 float32 micrographs at 1.0 A
-(300 kV, Cs 2.7 mm, amplitude contrast 0.07; defocus, astigmatism and azimuth
-vary per micrograph), already resident in HBM.  One step = one pass of the
-full plain path over the batch: Fourier-crop rescale to 3412^2 at 1.2 A,
-144-tile averaged amplitude spectrum, background removal, low-pass, radial
-average, 1D and 2D defocus/astigmatism search, spectral-CC resolution.
+(300 kV, Cs 2.7 mm, amplitude contrast 0.07; defocus, astigmatism, azimuth,
+B-factor and SNR vary per micrograph), already resident in HBM.  One step =
+one pass over the batch of:
+  learned (default)  rescale to 3412^2 at 1.2 A, 144-tile averaged spectrum,
+                     background, low-pass, radial average, proposal network,
+                     narrowed 2D search (plain search for rejected proposals),
+                     spectral-CC resolution
+  plain              the same spectrum stages, then the reference's 1D + 2D search
+  grid               the spectrum stages, then the 256 x 32 x 36 BF16 grid search
 
-Prints one JSON line (see the fields below).  Extra objects:
-  roofline      the dominant HBM-bound kernel, timed with HIP events on the
-                library's stream during one extra pass
+Prints one JSON line.  `value` is the selected path's micrographs/s; in
+learned mode the line also carries the plain path's rate on the same batch
+and the learned path's defocus agreement with the plain path over a held-out
+evaluation set.  Extra objects:
+  roofline      the dominant kernel, timed with HIP events on the library's
+                stream during one extra pass
   cpu_baseline  the CPU oracle on a bounded sample of the same workload
 """
 import argparse
@@ -35,8 +42,12 @@ def parse():
     ap.add_argument("--pix", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--mode", choices=["plain", "grid"], default="plain",
-                    help="plain: Do1D + Do2D search; grid: 256x32x36 BF16 MFMA grid search")
+    ap.add_argument("--mode", choices=["learned", "plain", "grid"], default="learned",
+                    help="learned: proposal network + narrowed search; plain: Do1D + Do2D search; "
+                         "grid: 256x32x36 BF16 MFMA grid search")
+    ap.add_argument("--eval", type=int, default=1024,
+                    help="learned mode: held-out micrographs for the defocus agreement (0: skip)")
+    ap.add_argument("--weights", default=None, help="proposal-network weights (.npz)")
     ap.add_argument("--chunk", type=int, default=0, help="images per spectrum chunk (0: library default)")
     ap.add_argument("--splits", type=int, default=0, help="tile-column splits (0: library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -57,18 +68,29 @@ def copy_calibration(torch, mib, steps):
           flush=True)
 
 
-def make_batch(torch, n, size, pix, seed0=1000):
+def make_batch(torch, n, size, pix, seed0=1000, out=None, wide=False):
+    """n seeded synthetic micrographs (micrograph i from seed seed0 + i).
+    Default: mean defocus log-uniform 8000-30000 A, astigmatism 0-1000 A, B
+    100, SNR 0.1.  wide: the proposal network's evaluation distribution --
+    defocus log-uniform 3000-50000 A, astigmatism 0-1500 A, B 50-200,
+    SNR 0.03-0.3."""
     from gctffind_amd import synth
     import numpy as np
-    rng = np.random.default_rng(seed0)
-    imgs = torch.empty((n, size, size), dtype=torch.float32, device="cuda:0")
+    imgs = out if out is not None else torch.empty((n, size, size), dtype=torch.float32, device="cuda:0")
     truth = []
     for i in range(n):
-        dfm = float(math.exp(rng.uniform(math.log(8000.0), math.log(30000.0))))
-        ast = float(rng.uniform(0.0, 1000.0))
+        rng = np.random.default_rng(seed0 + i)
+        if wide:
+            dfm = float(math.exp(rng.uniform(math.log(3000.0), math.log(50000.0))))
+            ast = float(rng.uniform(0.0, 1500.0))
+            bfac, snr = float(rng.uniform(50.0, 200.0)), float(math.exp(rng.uniform(math.log(0.03), math.log(0.3))))
+        else:
+            dfm = float(math.exp(rng.uniform(math.log(8000.0), math.log(30000.0))))
+            ast = float(rng.uniform(0.0, 1000.0))
+            bfac, snr = 100.0, 0.1
         az = float(rng.uniform(-90.0, 90.0))
         synth.micrograph(size, size, pix=pix, seed=seed0 + i, df1=dfm + ast / 2, df2=dfm - ast / 2,
-                         az_deg=az, device="cuda:0", out=imgs[i])
+                         az_deg=az, bfac=bfac, snr=snr, device="cuda:0", out=imgs[i])
         truth.append((dfm, ast, az))
     torch.cuda.synchronize()
     return imgs, truth
@@ -88,25 +110,24 @@ def main():
 
     ctx = gctf.Context()
     ctx.tune(chunk=a.chunk, tile_splits=a.splits)
-    imgs, truth = make_batch(torch, a.batch, a.size, a.pix)
     n = a.batch
+    learned = a.mode == "learned"
+    if learned:
+        from gctffind_amd import proposal
+        params, meta = proposal.load_params(a.weights or proposal.DEFAULT_WEIGHTS)
+        ctx.set_proposal_params(params)
+    imgs, truth = make_batch(torch, n, a.size, a.pix, seed0=2000 if learned else 1000, wide=learned)
     ptr = imgs.data_ptr()
 
     grid = dict(gctf.GRID_256x32x36)
+    plain_run = lambda: ctx.estimate_batch(ptr, n, a.size, a.size, a.pix)
     if a.mode == "grid":
         run = lambda: ctx.grid_search_batch(ptr, n, a.size, a.size, a.pix, grid)
+    elif learned:
+        run = lambda: ctx.estimate_batch_learned(ptr, n, a.size, a.size, a.pix)
     else:
-        run = lambda: ctx.estimate_batch(ptr, n, a.size, a.size, a.pix)
-    for _ in range(a.warmup):
-        res = run()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        res = run()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms = dt * 1000.0 / a.steps
-    value = n * a.steps / dt
+        run = plain_run
+    value, ms, res = timed(torch, run, n, a.steps, a.warmup)
 
     out = {
         "metric": "micrographs/s at 4096², learned-proposal vs plain search, |Δdefocus|≤20 Å",
@@ -132,15 +153,77 @@ def main():
             "path": a.mode,
         },
     }
+    if learned:
+        out["config"]["workload"] = (
+            f"learned-proposal CTF estimation (rescale + 144-tile spectrum + radial average + proposal "
+            f"network + +-300 A narrowed 2D search, plain search below confidence 0.8), batch of {n} "
+            f"synthetic {a.size}x{a.size} f32 micrographs at {a.pix} A")
+        pv, pms, pres = timed(torch, plain_run, n, a.steps, a.warmup)
+        out["config"]["plain_value"] = round(pv, 2)
+        out["config"]["plain_ms_per_step"] = round(pms, 3)
+        out["config"]["learned_over_plain"] = round(value / pv, 3)
+        out["config"]["proposal_weights"] = {k: meta.get(k) for k in ("spectra", "micrograph", "epochs", "steps")}
     dfe = [abs(0.5 * (r["df_min"] + r["df_max"]) - t[0]) for r, t in zip(res, truth)]
     out["config"]["median_abs_ddf_vs_truth_A"] = round(float(sorted(dfe)[len(dfe) // 2]), 1)
 
     if not a.no_roofline:
         out["roofline"], out["kernels"] = roofline(ctx, ptr, n, a, run)
+    img0 = imgs[0].cpu().numpy()
+    if learned and a.eval:
+        out["config"]["eval"] = evaluate(torch, ctx, a, imgs)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"], par = cpu_baseline(imgs[0].cpu().numpy(), a.pix, res[0])
+        out["cpu_baseline"], par = cpu_baseline(img0, a.pix, pres[0] if learned else res[0])
         out["config"]["parity_vs_oracle_img0"] = par
     print(json.dumps(out), flush=True)
+
+
+def timed(torch, run, n, steps, warmup):
+    """warmup untimed passes, then `steps` timed passes between device syncs."""
+    res = None
+    for _ in range(warmup):
+        res = run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = run()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return n * steps / dt, dt * 1000.0 / steps, res
+
+
+def evaluate(torch, ctx, a, buf):
+    """Learned path against the plain path (bitwise the FP64 oracle's
+    algorithm) on a.eval held-out micrographs, seeds 2000..: mean-defocus
+    and per-axis defocus differences, fallback and confidence rates, and
+    both paths' error against the generating defocus."""
+    import numpy as np
+    n = buf.shape[0]
+    dmean, daxis, fb, conf, e_l, e_p = [], [], [], [], [], []
+    for i0 in range(0, a.eval, n):
+        m = min(n, a.eval - i0)
+        _, truth = make_batch(torch, m, a.size, a.pix, seed0=2000 + i0, out=buf, wide=True)
+        lr, props = ctx.estimate_batch_learned(buf.data_ptr(), n, a.size, a.size, a.pix, proposals=True)
+        pr = ctx.estimate_batch(buf.data_ptr(), n, a.size, a.size, a.pix)
+        for j in range(m):
+            ml = 0.5 * (lr[j]["df_min"] + lr[j]["df_max"])
+            mp = 0.5 * (pr[j]["df_min"] + pr[j]["df_max"])
+            dmean.append(abs(ml - mp))
+            daxis.append(max(abs(lr[j]["df_min"] - pr[j]["df_min"]), abs(lr[j]["df_max"] - pr[j]["df_max"])))
+            fb.append(props[j]["fallback"])
+            conf.append(props[j]["confidence"])
+            e_l.append(abs(ml - truth[j][0]))
+            e_p.append(abs(mp - truth[j][0]))
+    dmean, daxis = np.array(dmean), np.array(daxis)
+    return {"n": len(dmean), "seeds": f"2000..{2000 + len(dmean) - 1}",
+            "reference": "plain GPU path (the reference algorithm, bitwise equal to the FP64 oracle on the parity cases)",
+            "frac_mean_df_within_20A": round(float(np.mean(dmean <= 20.0)), 4),
+            "frac_df_min_max_within_20A": round(float(np.mean(daxis <= 20.0)), 4),
+            "p99_abs_dmean_A": round(float(np.percentile(dmean, 99)), 1),
+            "max_abs_dmean_A": round(float(dmean.max()), 1),
+            "fallback_frac": round(float(np.mean(fb)), 4),
+            "median_confidence": round(float(np.median(conf)), 3),
+            "median_abs_ddf_vs_truth_learned_A": round(float(np.median(e_l)), 1),
+            "median_abs_ddf_vs_truth_plain_A": round(float(np.median(e_p)), 1)}
 
 
 def geometry(ctx, size, pix, tile=512):

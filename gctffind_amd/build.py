@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgctf_hip.so")
 ARCH = os.environ.get("GCTF_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["gctf_engine.hip"]
+SOURCES = ["gctf_engine.hip", "gctf_net.hip"]
 HEADERS = ["gctf_fft.hpp", "gctf_internal.hpp", "k_spectrum.hpp", "k_search.hpp"]
 
 

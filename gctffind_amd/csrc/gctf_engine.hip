@@ -23,6 +23,7 @@
 #include "k_spectrum.hpp"
 #include "k_search.hpp"
 #include "k_grid.hpp"
+#include "gctf_net.hpp"
 
 using namespace gctf;
 
@@ -177,6 +178,10 @@ struct gctf_ctx {
 	DevBuf raw, hsum, spect, mom, q, radial, ring_spec, ring_ss, states, part2d, refine, shell_cc, fullsp,
 		moments1;
 	DevBuf g_bt, g_fn, g_gt, g_cc, g_part, g_top, g_rs, g_rescored;
+	// learned path: proposal-network parameters, activations, outputs
+	DevBuf net_params, net_out, props;
+	bool have_net = false;
+	NetBuffers* netb = nullptr;
 	std::vector<SearchState> h_states;
 	SearchState* h_pinned = nullptr;
 	int h_pinned_n = 0;
@@ -193,6 +198,7 @@ struct gctf_ctx {
 		for (auto& kv : graphs)
 			if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
 		if (h_pinned) (void)hipHostFree(h_pinned);
+		net_buffers_destroy(netb);
 		for (auto& r : prec) { evpool.push_back(r.a); evpool.push_back(r.b); }
 		for (auto e : evpool) (void)hipEventDestroy(e);
 		if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -732,7 +738,8 @@ int launch_2d(gctf_ctx* c, const SearchPlan& sp, int n, int begin, float phase_r
 		PROF(c, "k_cc2d_eval", k_cc2d_eval<CPB><<<grid, 256, 0, s>>>(sp.cc, sp.ring, c->ring_spec.as<float>(), st, part));
 	};
 	auto step = [&](int next) {
-		PROF(c, "k_s2d_step", k_s2d_step<<<n, 256, 0, s>>>(st, part, ss, next, c->refine.as<RefineArgs>()));
+		PROF(c, "k_s2d_step", k_s2d_step<<<n, 256, 0, s>>>(st, part, ss, next, c->refine.as<RefineArgs>(),
+			plain_program()));
 	};
 	int maxph = 61;
 	if (phase_range > 61.0f) maxph = std::min(kMaxCands, (int)phase_range + 3);
@@ -758,12 +765,121 @@ int launch_2d(gctf_ctx* c, const SearchPlan& sp, int n, int begin, float phase_r
 	return GCTF_OK;
 }
 
+// ---- learned path --------------------------------------------------------
+// Network outputs -> proposal (mean defocus A, astigmatism A, azimuth deg,
+// confidence) and the defocus window of the narrowed search.
+__global__ void k_proposal(const float* __restrict__ o, int n, float window, float* __restrict__ prop,
+	RefineArgs* __restrict__ refine, SearchState* __restrict__ st)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const float* oi = o + i * kNetOut;
+	const float df = 1.0e4f * expf(fminf(fmaxf(oi[0], -3.0f), 3.0f));
+	float az = oi[2] * 57.29577951308232f;
+	az = az - 180.0f * floorf((az + 90.0f) / 180.0f);   // [-90, 90)
+	const float conf = 1.0f / (1.0f + expf(-oi[3]));
+	prop[i * 4 + 0] = df;
+	prop[i * 4 + 1] = 1.0e3f * oi[1];
+	prop[i * 4 + 2] = az;
+	prop[i * 4 + 3] = conf;
+	RefineArgs r;
+	r.dfmean[0] = df; r.dfmean[1] = 2.0f * window;
+	r.ast[0] = 0.0f; r.ast[1] = 0.0f;
+	r.ang[0] = 0.0f; r.ang[1] = 0.0f;
+	r.phase[0] = 0.0f; r.phase[1] = 0.0f;
+	refine[i] = r;
+	st[i].conf = conf;
+}
+
+// Hand the micrograph to the plain search when the proposal is not trusted
+// (confidence below the threshold), the narrowed search failed (score <= 0,
+// the reference's failure signal), or its defocus ended within one fine step
+// of the window edge (the optimum may lie outside).
+__global__ void k_learned_check(SearchState* __restrict__ st, int n, float conf_thr)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	SearchState& s = st[i];
+	const float edge = fmaxf(s.pg.fine_step, 1.0f);
+	const bool at_lo = s.dfrange[0] > 3000.0f && s.dfmean - s.dfrange[0] < edge;
+	const bool at_hi = s.dfrange[1] - s.dfmean < edge;
+	s.fallback = (s.conf < conf_thr || !(s.score > 0.0f) || at_lo || at_hi) ? 1 : 0;
+}
+
+// Before the fallback pass: images the learned path settled sit out; the
+// others restart from CFindCtfBase::SetPhase's values.
+__global__ void k_fallback_prepare(SearchState* __restrict__ st, int n, float p0, float range)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	SearchState& s = st[i];
+	if (!s.fallback) { s.skip = 1; return; }
+	s.skip = 0;
+	s.ext_phase = p0;
+	s.phase_range = range;
+}
+
+__global__ void k_clear_skip(SearchState* __restrict__ st, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) st[i].skip = 0;
+}
+
+// The learned path's 2D program: the mIterate stage sequence with the
+// counts of `pg`, in the proposal's window, then the fine defocus stage.
+int launch_2d_learned(gctf_ctx* c, const SearchPlan& sp, int n, const SearchProgram& pg, float phase_range)
+{
+	hipStream_t s = c->stream;
+	const int T = sp.T, cx = T / 2 + 1;
+	SearchState* st = c->states.as<SearchState>();
+	double2* part = c->part2d.as<double2>();
+	const double* ss = c->ring_ss.as<double>();
+	PROF(c, "k_ring_gather", k_ring_gather<<<n, 1024, 0, s>>>(c->spect.as<float>(), T * cx, sp.ring,
+		c->ring_spec.as<float>(), c->ring_ss.as<double>()));
+	constexpr int CPB = 8;
+	auto eval = [&](int maxc) {
+		dim3 grid((maxc + CPB - 1) / CPB, kPixChunks, n);
+		PROF(c, "k_cc2d_eval", k_cc2d_eval<CPB><<<grid, 256, 0, s>>>(sp.cc, sp.ring, c->ring_spec.as<float>(), st, part));
+	};
+	auto step = [&](int next) {
+		PROF(c, "k_s2d_step", k_s2d_step<<<n, 256, 0, s>>>(st, part, ss, next, c->refine.as<RefineArgs>(), pg));
+	};
+	int maxph = 61;
+	if (phase_range > 61.0f) maxph = std::min(kMaxCands, (int)phase_range + 3);
+	const int last = pg.fine_n >= 2 ? NX_FINE : NX_FINISH;
+	step(NX_BEGIN_PROPOSAL);
+	eval(1);
+	step(NX_ASTIG);
+	eval(pg.n_ast * pg.n_ang);
+	step(NX_ASTMAG);
+	for (int i = 1; i < pg.iters; ++i) {
+		eval(pg.n_mag + 1);
+		step(NX_ASTANG);
+		eval(pg.n_angl + 1);
+		step(NX_DFMEAN);
+		eval(pg.n_df + 1);
+		step(NX_PHASE);
+		if (phase_range > 0.0f) eval(maxph);
+		step(i < pg.iters - 1 ? NX_ASTMAG : last);
+	}
+	if (last == NX_FINE) {
+		eval(pg.fine_n + 1);
+		step(NX_FINISH);
+	}
+	PROF(c, "k_spectral_cc", k_spectral_cc<<<dim3(cx, n), 256, 0, s>>>(sp.cc, cx, T, 10, c->spect.as<float>(), st,
+		c->shell_cc.as<float>()));
+	PROF(c, "k_ctf_res", k_ctf_res<<<(n + 63) / 64, 64, 0, s>>>(c->shell_cc.as<float>(), cx, T, sp.pix, st, n));
+	HIPCHK(c, hipGetLastError());
+	return GCTF_OK;
+}
+
 __global__ void k_state_init(SearchState* st, int n, float p0, float range)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	SearchState s;
 	memset(&s, 0, sizeof s);
+	s.pg = plain_program();
 	s.ext_phase = p0;          // CFindCtfBase::SetPhase
 	s.phase_range = range;
 	st[i] = s;
@@ -1233,11 +1349,16 @@ namespace {
 
 // Search variants of a batch after the shared spectrum stages.
 struct BatchJob {
-	int mode = 0;                 // 0: Do1D + Do2D, 1: BF16 grid search
+	int mode = 0;                 // 0: Do1D + Do2D, 1: BF16 grid search, 2: learned, 3: radial only
 	GridSpec gs{};                // mode 1
 	SearchPlan* gring = nullptr;  // mode 1: ring of the grid correlation
 	int np = 0;                   // mode 1: micrograph columns padded to 32
 	int nchunks = 0;              // mode 1: top-k row chunks
+	SearchProgram pg{};           // mode 2: narrowed search program
+	float window = 300.0f;        // mode 2: half-width of the defocus window (A)
+	float conf_thr = 0.8f;        // mode 2: proposal confidence threshold
+	float* radial_out = nullptr;  // mode 3: device [n][ld]
+	int radial_ld = 0;
 };
 
 int record_spectra(gctf_ctx* c, const RescalePlan& rp, TileTables& tt, const void* imgs, int dtype, int n,
@@ -1301,7 +1422,7 @@ int launch_grid(gctf_ctx* c, const BatchJob& j, int n)
 	const int rows = (j.gs.m + j.nchunks - 1) / j.nchunks;
 	PROF(c, "k_grid_topk", k_grid_topk<<<j.nchunks, 256, 0, s>>>(c->g_cc.as<float>(), j.gs.m, j.np, n, rows,
 		c->g_part.as<float2>()));
-	PROF(c, "k_grid_merge", k_grid_merge<<<n, 64, 0, s>>>(c->g_part.as<float2>(), j.nchunks, j.np,
+	PROF(c, "k_grid_merge", k_grid_merge<<<n, 256, 0, s>>>(c->g_part.as<float2>(), j.nchunks, j.np,
 		c->g_top.as<int>()));
 	PROF(c, "k_grid_rescore", k_grid_rescore<<<dim3(kGridTopK / 8, kPixChunks, n), 256, 0, s>>>(j.gs, r.cc,
 		r.ring, c->ring_spec.as<float>(), c->g_top.as<int>(), c->prm.ext_phase_deg, c->g_rs.as<double2>()));
@@ -1317,6 +1438,25 @@ int record_batch(gctf_ctx* c, const RescalePlan& rp, SearchPlan& sp, TileTables&
 {
 	CHK(record_spectra(c, rp, tt, imgs, dtype, n, stride));
 	if (job.mode == 1) return launch_grid(c, job, n);
+	if (job.mode == 2 || job.mode == 3) {
+		const int T = sp.T, cx = T / 2 + 1;
+		hipStream_t s = c->stream;
+		PROF(c, "k_radial", k_radial<<<n, 256, 0, s>>>(c->spect.as<float>(), cx, T, tt.roff.as<int>(),
+			tt.rcnt.as<int>(), tt.ridx.as<int>(), c->radial.as<float>()));
+		if (job.mode == 3) {
+			HIPCHK(c, hipMemcpy2DAsync(job.radial_out, sizeof(float) * job.radial_ld, c->radial.p,
+				sizeof(float) * cx, sizeof(float) * std::min(cx, job.radial_ld), n, hipMemcpyDeviceToDevice, s));
+			return GCTF_OK;
+		}
+		HIPCHK(c, net_forward(c->netb, c->net_params.as<float>(), c->radial.as<float>(), cx, n, true,
+			c->net_out.as<float>(), s));
+		PROF(c, "k_proposal", k_proposal<<<(n + 63) / 64, 64, 0, s>>>(c->net_out.as<float>(), n, job.window,
+			c->props.as<float>(), c->refine.as<RefineArgs>(), c->states.as<SearchState>()));
+		CHK(launch_2d_learned(c, sp, n, job.pg, std::max(c->prm.ext_phase_range_deg, 0.0f)));
+		PROF(c, "k_learned_check", k_learned_check<<<(n + 63) / 64, 64, 0, s>>>(c->states.as<SearchState>(), n,
+			job.conf_thr));
+		return GCTF_OK;
+	}
 	CHK(launch_1d(c, sp, tt, n, 0, 0.f, 0.f));
 	CHK(launch_2d(c, sp, n, NX_BEGIN_DO2D, std::max(c->prm.ext_phase_range_deg, 0.0f)));
 	return GCTF_OK;
@@ -1349,6 +1489,17 @@ int run_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int
 	if (rp.active) {
 		CHK(ensure_rescale_buffers(ctx, rp, C));
 		CHK(ensure_buf(ctx, ctx->padc, sizeof(float) * (size_t)rp.pitch * rp.nny * C));
+	}
+	if (job.mode == 2) {
+		if (!ctx->have_net) return fail(ctx, GCTF_ERR_STATE, "no proposal-network parameters installed");
+		if (!ctx->netb || net_buffers_capacity(ctx->netb) < n) {
+			net_buffers_destroy(ctx->netb);
+			ctx->netb = nullptr;
+			HIPCHK(ctx, net_buffers_create(n, false, &ctx->netb));
+			ctx->gen++;
+		}
+		CHK(ensure_buf(ctx, ctx->net_out, sizeof(float) * kNetOut * n));
+		CHK(ensure_buf(ctx, ctx->props, sizeof(float) * 4 * n));
 	}
 	if (!ctx->use_graph || ctx->prof) {
 		CHK(record_batch(ctx, rp, *sp, *tt, dev_imgs, dtype, n, stride, job));
@@ -1408,7 +1559,7 @@ int gctf_grid_search_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n
 	g.az_step = (grid->az_hi - grid->az_lo) / (float)grid->n_az;
 	g.m = (int)m;
 	job.np = (n + 31) / 32 * 32;
-	job.nchunks = 256;
+	job.nchunks = (int)std::min<long long>(4096, (m + 255) / 256);   // about 256 rows per top-k thread
 	// the ring plan of the grid correlation (resolution bounds in A)
 	HIPCHK(ctx, hipSetDevice(ctx->device));
 	RescalePlan rp;
@@ -1443,3 +1594,109 @@ int gctf_grid_search_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n
 }
 
 
+
+int gctf_radial_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny, long long stride,
+	float pix, float* d_radial_out, int ld)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	if (!d_radial_out || ld < 1) return fail(ctx, GCTF_ERR_ARG, "null radial output or ld < 1");
+	if (n < 1) return fail(ctx, GCTF_ERR_ARG, "n < 1");
+	BatchJob job;
+	job.mode = 3;
+	job.radial_out = d_radial_out;
+	job.radial_ld = ld;
+	std::vector<gctf_result> tmp((size_t)n);
+	char key[96];
+	snprintf(key, sizeof key, "radial/%p/%d", (void*)d_radial_out, ld);
+	return run_batch(ctx, dev_imgs, dtype, n, nx, ny, stride, pix, tmp.data(), job, key);
+}
+
+int gctf_set_proposal_params(gctf_ctx* ctx, const float* host_params, long long count)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	if (!host_params || count != kNetParams) return fail(ctx, GCTF_ERR_ARG, "proposal parameter count");
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	CHK(ensure_buf(ctx, ctx->net_params, sizeof(float) * kNetParams));
+	HIPCHK(ctx, hipMemcpy(ctx->net_params.p, host_params, sizeof(float) * kNetParams, hipMemcpyHostToDevice));
+	ctx->have_net = true;
+	return GCTF_OK;
+}
+
+int gctf_estimate_batch_learned(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny,
+	long long stride, float pix, const gctf_search_program* prog, float conf_threshold, float window_A,
+	gctf_result* out, gctf_proposal* prop_out)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	BatchJob job;
+	job.mode = 2;
+	SearchProgram& pg = job.pg;
+	if (prog) {
+		pg.n_ast = prog->n_ast; pg.n_ang = prog->n_ang; pg.n_mag = prog->n_mag; pg.n_angl = prog->n_angl;
+		pg.n_df = prog->n_df; pg.iters = prog->iters; pg.df_minstep = prog->df_minstep;
+		pg.fine_n = prog->fine_n; pg.fine_step = prog->fine_step;
+	} else {
+		gctf_search_program d;
+		gctf_learned_program_default(&d);
+		pg.n_ast = d.n_ast; pg.n_ang = d.n_ang; pg.n_mag = d.n_mag; pg.n_angl = d.n_angl;
+		pg.n_df = d.n_df; pg.iters = d.iters; pg.df_minstep = d.df_minstep;
+		pg.fine_n = d.fine_n; pg.fine_step = d.fine_step;
+	}
+	if (pg.n_ast < 1 || pg.n_ang < 1 || pg.n_ast * pg.n_ang > kMaxCands || pg.n_mag < 2 || pg.n_angl < 2
+		|| pg.n_df < 2 || pg.n_mag >= kMaxCands || pg.n_angl >= kMaxCands || pg.n_df >= kMaxCands
+		|| pg.iters < 2 || pg.iters > 200 || !(pg.df_minstep > 0) || pg.fine_n < 0 || pg.fine_n >= kMaxCands
+		|| (pg.fine_n >= 2 && !(pg.fine_step > 0)))
+		return fail(ctx, GCTF_ERR_ARG, "search program outside the supported counts");
+	if (!(window_A > 0)) return fail(ctx, GCTF_ERR_ARG, "window must be positive");
+	job.window = window_A;
+	job.conf_thr = conf_threshold;
+	char key[192];
+	snprintf(key, sizeof key, "learned/%d/%d/%d/%d/%d/%d/%a/%d/%a/%a/%a", pg.n_ast, pg.n_ang, pg.n_mag,
+		pg.n_angl, pg.n_df, pg.iters, pg.df_minstep, pg.fine_n, pg.fine_step, window_A, conf_threshold);
+	CHK(run_batch(ctx, dev_imgs, dtype, n, nx, ny, stride, pix, out, job, key));
+	// fallback pass: the plain search on the micrographs the learned path did not settle
+	std::vector<int> fb((size_t)n);
+	int nfb = 0;
+	for (int i = 0; i < n; ++i) { fb[i] = ctx->h_pinned[i].fallback; nfb += fb[i]; }
+	if (nfb > 0) {
+		RescalePlan rp;
+		CHK(make_rescale_plan(ctx, nx, ny, pix, rp));
+		SearchPlan* sp;
+		TileTables* tt;
+		CHK(get_search_plan(ctx, ctx->prm.kv, ctx->prm.cs_mm, ctx->prm.amp_contrast, rp.pixn,
+			ctx->prm.tile_size, &sp));
+		CHK(get_tile_tables(ctx, ctx->prm.tile_size, &tt));
+		const float range = std::max(ctx->prm.ext_phase_range_deg, 0.0f);
+		SearchState* st = ctx->states.as<SearchState>();
+		k_fallback_prepare<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n, ctx->prm.ext_phase_deg, range);
+		CHK(launch_1d(ctx, *sp, *tt, n, 0, 0.f, 0.f));
+		CHK(launch_2d(ctx, *sp, n, NX_BEGIN_DO2D, range));
+		k_clear_skip<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n);
+		HIPCHK(ctx, hipGetLastError());
+		CHK(read_results(ctx, n, out));
+	}
+	if (prop_out) {
+		std::vector<float> pr((size_t)4 * n);
+		HIPCHK(ctx, hipMemcpy(pr.data(), ctx->props.p, sizeof(float) * 4 * n, hipMemcpyDeviceToHost));
+		for (int i = 0; i < n; ++i) {
+			prop_out[i].df_mean = pr[4 * i];
+			prop_out[i].astig = pr[4 * i + 1];
+			prop_out[i].azimuth_deg = pr[4 * i + 2];
+			prop_out[i].confidence = pr[4 * i + 3];
+			prop_out[i].fallback = fb[i];
+		}
+	}
+	return GCTF_OK;
+}
+
+int gctf_learned_program_default(gctf_search_program* out)
+{
+	if (!out) return GCTF_ERR_ARG;
+	out->n_ast = 17; out->n_ang = 26;
+	out->n_mag = 11; out->n_angl = 27;
+	out->n_df = 11;
+	out->iters = 10;
+	out->df_minstep = 20.0f;
+	out->fine_n = 9;
+	out->fine_step = 5.0f;
+	return GCTF_OK;
+}

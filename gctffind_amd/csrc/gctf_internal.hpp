@@ -33,6 +33,26 @@ struct CtfConsts {
 	float pix;        // working pixel size (A)
 };
 
+// Candidate counts of the 2D search program.  The plain program is the
+// reference's (CFindDefocus2D.cpp:153-361): a 51 x 51 astigmatism grid, then
+// 19 rounds of 21 / 51 / 31 candidates, defocus step floor 20 A.  The
+// learned path runs a smaller program inside the proposal's window and
+// ends with a fine defocus stage.
+struct SearchProgram {
+	int n_ast, n_ang;     // mFindAstig grid
+	int n_mag, n_angl;    // mRefineAstMag / mRefineAstAng candidates
+	int n_df;             // mRefineDfMean candidates
+	int iters;            // mIterate's iIterations
+	float df_minstep;     // mRefineDfMean step floor (A)
+	int fine_n;           // final defocus stage (0: none)
+	float fine_step;      // its step (A)
+};
+
+__host__ __device__ constexpr SearchProgram plain_program()
+{
+	return SearchProgram{51, 51, 21, 51, 31, 20, 20.0f, 0, 0.0f};
+}
+
 // Per-micrograph search state; mirrors the fields of CFindCtfBase,
 // CFindDefocus1D and CFindDefocus2D (CFindCTFInc.h:480-562).
 struct SearchState {
@@ -56,6 +76,10 @@ struct SearchState {
 	float st_lo, st_hi;      // phase window for ST_PHASE
 	float st_old;            // mRefineDfMean: the df before the stage
 	int lowpassed;           // a12 gate outcome
+	SearchProgram pg;        // candidate counts of the running 2D program
+	int skip;                // 1: the image sits out the running search pass
+	int fallback;            // learned path: 1 when the plain search must decide
+	float conf;              // learned path: proposal confidence
 	int pad_;
 };
 

@@ -257,49 +257,48 @@ __global__ __launch_bounds__(256) void k_grid_topk(const float* __restrict__ cc,
 	for (int i = 0; i < kGridTopK; ++i) out[i] = make_float2(v[i], __int_as_float(ix[i]));
 }
 
-// (cc, m) ordering of the top-k lists: larger cc first, then smaller m.
-__device__ __forceinline__ void topk_insert(float* v, int* ix, float cv, int ci)
+// (cc, m) total order of the candidate lists: larger cc first, then smaller m.
+__device__ __forceinline__ bool topk_better(float2 x, float2 y)
 {
+	return x.x > y.x || (x.x == y.x && __float_as_int(x.y) < __float_as_int(y.y));
+}
+
+// dst <- the best kGridTopK of the sorted lists dst and src (both sorted
+// under topk_better, padded with (-3e38, INT_MAX)).
+__device__ __forceinline__ void topk_merge(float2* dst, const float2* src)
+{
+	float2 o[kGridTopK];
+	int i = 0, j = 0;
 #pragma unroll
-	for (int i = 0; i < kGridTopK; ++i) {
-		if (cv > v[i] || (cv == v[i] && ci < ix[i])) {
-			float tv = v[i]; int ti = ix[i];
-			v[i] = cv; ix[i] = ci;
-			cv = tv; ci = ti;
-		}
+	for (int q = 0; q < kGridTopK; ++q) {
+		const float2 x = dst[i], y = src[j];
+		const bool take = topk_better(x, y);
+		o[q] = take ? x : y;
+		i += take ? 1 : 0;
+		j += take ? 0 : 1;
 	}
+#pragma unroll
+	for (int q = 0; q < kGridTopK; ++q) dst[q] = o[q];
 }
 
 // Merge the chunk lists of one micrograph into its top-k candidate indices:
-// 64 threads merge interleaved chunks, thread 0 merges their 64 lists.
-__global__ __launch_bounds__(64) void k_grid_merge(const float2* __restrict__ part, int nchunks, int np,
+// 256 threads each fold a strided set of chunk lists into one list in LDS,
+// then a tree of pairwise merges.
+__global__ __launch_bounds__(256) void k_grid_merge(const float2* __restrict__ part, int nchunks, int np,
 	int* __restrict__ top)
 {
-	__shared__ float sv[64][kGridTopK];
-	__shared__ int si[64][kGridTopK];
+	__shared__ float2 sl[256][kGridTopK];
 	const int col = blockIdx.x, t = threadIdx.x;
-	float v[kGridTopK];
-	int ix[kGridTopK];
+	float2* mine = sl[t];
 #pragma unroll
-	for (int i = 0; i < kGridTopK; ++i) { v[i] = -3.0e38f; ix[i] = 0x7fffffff; }
-	for (int c = t; c < nchunks; c += 64) {
-		const float2* p = part + ((long long)c * np + col) * kGridTopK;
-		for (int q = 0; q < kGridTopK; ++q) {
-			const int ci = __float_as_int(p[q].y);
-			if (ci == 0x7fffffff) break;
-			topk_insert(v, ix, p[q].x, ci);
-		}
-	}
-#pragma unroll
-	for (int i = 0; i < kGridTopK; ++i) { sv[t][i] = v[i]; si[t][i] = ix[i]; }
+	for (int q = 0; q < kGridTopK; ++q) mine[q] = make_float2(-3.0e38f, __int_as_float(0x7fffffff));
+	for (int c = t; c < nchunks; c += 256) topk_merge(mine, part + ((long long)c * np + col) * kGridTopK);
 	__syncthreads();
-	if (t != 0) return;
-	for (int u = 1; u < 64; ++u)
-		for (int q = 0; q < kGridTopK; ++q) {
-			if (si[u][q] == 0x7fffffff) break;
-			topk_insert(v, ix, sv[u][q], si[u][q]);
-		}
-	for (int i = 0; i < kGridTopK; ++i) top[col * kGridTopK + i] = ix[i];
+	for (int s = 128; s > 0; s >>= 1) {
+		if (t < s) topk_merge(mine, sl[t + s]);
+		__syncthreads();
+	}
+	if (t < kGridTopK) top[col * kGridTopK + t] = __float_as_int(sl[0][t].y);
 }
 
 // FP32 re-score of the top-k candidates with the 2D-search evaluator;

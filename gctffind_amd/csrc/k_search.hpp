@@ -131,6 +131,7 @@ __global__ __launch_bounds__(256) void k_search1d(Search1DParams p, const float*
 	__shared__ int s_i[256];
 	__shared__ float res[3];
 	SearchState* S = st + blockIdx.x;
+	if (S->skip) return;
 	const float* rad = radial + (long long)blockIdx.x * p.cmpx;
 	const float pix = p.c.pix;
 	float df_min, ext_phase;
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(256) void k_cc2d_eval(CtfConsts cc, RingTables ring
 	__shared__ int cvalid[CPB];
 	__shared__ double red[2][CPB][4];
 	const SearchState& S = st[blockIdx.z];
-	if (S.done || S.st_kind == ST_NONE) return;
+	if (S.skip || S.done || S.st_kind == ST_NONE) return;
 	const int n = S.st_n;
 	const int c0 = blockIdx.x * CPB;
 	if (c0 >= n) return;
@@ -318,7 +319,7 @@ __device__ void prep_astig(SearchState& s)
 	float astr = s.astrange[1] - s.astrange[0];
 	float angr = s.angrange[1] - s.angrange[0];
 	if (astr < tiny && angr < tiny) { prep_none(s); return; }
-	int nast = 51, nang = 51;
+	int nast = s.pg.n_ast, nang = s.pg.n_ang;
 	float aststep = astr / (float)nast;
 	float angstep = angr / (float)nang;
 	if ((double)aststep < 1e-5) nast = 1;
@@ -352,11 +353,11 @@ __device__ void prep_dfmean(SearchState& s, float range)
 {
 	const float tiny = (float)1e-30;
 	if (range < tiny) { prep_none(s); return; }
-	int n = 31;
+	int n = s.pg.n_df;
 	float mn = fmaxf(s.dfmean - range * 0.5f, s.dfrange[0]);
 	float mx = fminf(s.dfmean + range * 0.5f, s.dfrange[1]);
 	float step = (mx - mn) / (float)n;
-	if (step < 20) step = 20.0f;
+	if (step < s.pg.df_minstep) step = s.pg.df_minstep;
 	n = (int)((mx - mn) / step) / 2 * 2 + 1;
 	if (n <= 1) { prep_none(s); return; }
 	s.st_kind = ST_DFMEAN; s.st_n = n; s.st_a0 = mn; s.st_astep = step;
@@ -384,9 +385,23 @@ __device__ void prep_phase(SearchState& s, float range)
 	s.st_kind = ST_PHASE; s.st_n = n; s.st_astep = step; s.st_lo = mnp; s.st_hi = mxp;
 }
 
-__device__ __forceinline__ float iter_scale(int i)
+__device__ __forceinline__ float iter_scale(const SearchState& s, int i)
 {
-	return 1.0f - (float)i * 0.5f / (float)20;   // CFindDefocus2D.cpp:167
+	return 1.0f - (float)i * 0.5f / (float)s.pg.iters;   // CFindDefocus2D.cpp:167
+}
+
+// Learned path's last stage: s.pg.fine_n defocus candidates s.pg.fine_step
+// apart around the current mean defocus, inside the window.
+__device__ void prep_fine(SearchState& s)
+{
+	if (s.pg.fine_n < 2) { prep_none(s); return; }
+	const int h = s.pg.fine_n / 2;
+	float mn = fmaxf(s.dfmean - (float)h * s.pg.fine_step, s.dfrange[0]);
+	float mx = fminf(s.dfmean + (float)h * s.pg.fine_step, s.dfrange[1]);
+	int n = (int)((mx - mn) / s.pg.fine_step) + 1;
+	if (n <= 1) { prep_none(s); return; }
+	s.st_kind = ST_DFMEAN; s.st_n = n; s.st_a0 = mn; s.st_astep = s.pg.fine_step;
+	s.st_old = s.dfmean;
 }
 
 // Stage program of mIterate; `next` names what the controller prepares.
@@ -399,6 +414,8 @@ enum NextStage : int {
 	NX_DFMEAN = 5,       // consume ASTANG, prepare DFMEAN
 	NX_PHASE = 6,        // consume DFMEAN, prepare PHASE
 	NX_FINISH = 7,       // consume PHASE of the last iteration
+	NX_BEGIN_PROPOSAL = 8, // learned path: window around the proposal, DoIt caps, then INIT
+	NX_FINE = 9,         // consume PHASE of the last iteration, prepare the fine defocus stage
 };
 
 struct RefineArgs {
@@ -410,16 +427,31 @@ struct RefineArgs {
 // state as the reference method does, and prepare the next stage.
 __global__ __launch_bounds__(256) void k_s2d_step(SearchState* __restrict__ st,
 	const double2* __restrict__ part, const double* __restrict__ ring_ss, int next,
-	const RefineArgs* __restrict__ refine)
+	const RefineArgs* __restrict__ refine, SearchProgram prog)
 {
 	__shared__ float s_cc[256];
 	__shared__ int s_i[256];
 	SearchState* S = st + blockIdx.x;
 	const int tid = threadIdx.x;
-	if (next == NX_BEGIN_DO2D || next == NX_BEGIN_REFINE) {
+	if (S->skip) return;
+	if (next == NX_BEGIN_DO2D || next == NX_BEGIN_REFINE || next == NX_BEGIN_PROPOSAL) {
 		if (tid == 0) {
 			SearchState s = *S;
-			if (next == NX_BEGIN_DO2D) {
+			s.pg = plain_program();
+			if (next == NX_BEGIN_PROPOSAL) {
+				// the reference's DoIt box (CFindDefocus2D.cpp:113-118) with the
+				// defocus window of the proposal
+				const RefineArgs& r = refine[blockIdx.x];
+				s.pg = prog;
+				s.dfmean = r.dfmean[0]; s.ast = 0.0f; s.ang = 0.0f;
+				s.phrange[0] = s.ext_phase;
+				s.phrange[1] = s.phase_range;
+				const float h = 0.5f * r.dfmean[1];
+				s.dfrange[0] = fmaxf(s.dfmean - h, 3000.0f);
+				s.dfrange[1] = s.dfmean + h;
+				s.astrange[0] = 0.0f; s.astrange[1] = 0.06f;
+				s.angrange[0] = -90.0f; s.angrange[1] = 90.0f;
+			} else if (next == NX_BEGIN_DO2D) {
 				// CFindCtf2D::Do2D (CFindCtf2D.cpp:44-53) + DoIt (:110-124)
 				float dfmean = (s.df_min + s.df_max) * 0.5f;
 				s.dfmean = dfmean; s.ast = 0.0f; s.ang = 0.0f;
@@ -529,19 +561,22 @@ __global__ __launch_bounds__(256) void k_s2d_step(SearchState* __restrict__ st,
 				s.tot_ang = s.angrange[1] - s.angrange[0];
 			}
 			s.iter += 1;
-			prep_line(s, ST_ASTMAG, s.ast, s.astrange, iter_scale(s.iter) * s.tot_ast, 21, 0.001f);
+			prep_line(s, ST_ASTMAG, s.ast, s.astrange, iter_scale(s, s.iter) * s.tot_ast, s.pg.n_mag, 0.001f);
 			break;
 		case NX_ASTANG:
 			// CFindDefocus2D.cpp:171-173
 			if (s.oldcc > s.ccmax) { s.done = 1; break; }
 			s.ccmax = s.oldcc;
-			prep_line(s, ST_ASTANG, s.ang, s.angrange, iter_scale(s.iter) * s.tot_ang, 51, 1.0f);
+			prep_line(s, ST_ASTANG, s.ang, s.angrange, iter_scale(s, s.iter) * s.tot_ang, s.pg.n_angl, 1.0f);
 			break;
 		case NX_DFMEAN:
-			prep_dfmean(s, iter_scale(s.iter) * s.tot_df);
+			prep_dfmean(s, iter_scale(s, s.iter) * s.tot_df);
 			break;
 		case NX_PHASE:
-			prep_phase(s, iter_scale(s.iter) * s.phrange[1]);
+			prep_phase(s, iter_scale(s, s.iter) * s.phrange[1]);
+			break;
+		case NX_FINE:
+			prep_fine(s);
 			break;
 		case NX_FINISH:
 			prep_none(s);
@@ -571,6 +606,7 @@ __global__ __launch_bounds__(256) void k_spectral_cc(CtfConsts cc, int cmpx, int
 {
 	__shared__ double red[6][256];
 	const SearchState& S = st[blockIdx.y];
+	if (S.skip) return;
 	const int k = blockIdx.x;
 	int lo = k - width / 2, hi = lo + width;
 	if (lo < 0) { lo = 0; hi = width; }
@@ -623,7 +659,7 @@ __global__ void k_ctf_res(const float* __restrict__ shell_cc, int cmpx, int cmpy
 	SearchState* __restrict__ st, int nimg)
 {
 	const int b = blockIdx.x * blockDim.x + threadIdx.x;
-	if (b >= nimg) return;
+	if (b >= nimg || st[b].skip) return;
 	const float* cc = shell_cc + (long long)b * cmpx;
 	int imax = 1;
 	float fmax = (float)-1e30;

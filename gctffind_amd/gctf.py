@@ -27,6 +27,11 @@ EXPORTS = [
     "gctf_refine", "gctf_spect_size", "gctf_get_half_spect", "gctf_set_half_spect",
     "gctf_full_spectrum", "gctf_estimate_batch", "gctf_tune", "gctf_profile",
     "gctf_profile_report", "gctf_grid_search_batch",
+    "gctf_learned_program_default", "gctf_radial_batch", "gctf_set_proposal_params",
+    "gctf_estimate_batch_learned",
+    "gctf_net_param_count", "gctf_net_create", "gctf_net_destroy", "gctf_net_error_string",
+    "gctf_net_set_params", "gctf_net_get_params", "gctf_net_reset_optimizer", "gctf_net_forward",
+    "gctf_net_loss_grad", "gctf_net_adam_step", "gctf_net_synchronize",
 ]
 
 
@@ -58,6 +63,23 @@ class GctfGrid(ctypes.Structure):
 # ring 1/30 - 1/4 1/A)
 GRID_256x32x36 = dict(df_lo=3000.0, df_hi=43000.0, n_df=256, ast_lo=0.0, ast_hi=0.06, n_ast=32,
                       az_lo=-90.0, az_hi=90.0, n_az=36, res_lo=30.0, res_hi=4.0)
+
+
+class GctfSearchProgram(ctypes.Structure):
+    _fields_ = [("n_ast", ctypes.c_int), ("n_ang", ctypes.c_int), ("n_mag", ctypes.c_int),
+                ("n_angl", ctypes.c_int), ("n_df", ctypes.c_int), ("iters", ctypes.c_int),
+                ("df_minstep", ctypes.c_float), ("fine_n", ctypes.c_int), ("fine_step", ctypes.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GctfProposal(ctypes.Structure):
+    _fields_ = [("df_mean", ctypes.c_float), ("astig", ctypes.c_float), ("azimuth_deg", ctypes.c_float),
+                ("confidence", ctypes.c_float), ("fallback", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class GctfError(RuntimeError):
@@ -105,6 +127,22 @@ def load():
         "gctf_profile_report": (ci, [vp, ci, fp, fp, fp, fp]),
         "gctf_grid_search_batch": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf,
                                         ctypes.POINTER(GctfGrid), fp, fp, fp, fp]),
+        "gctf_learned_program_default": (ci, [ctypes.POINTER(GctfSearchProgram)]),
+        "gctf_radial_batch": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, vp, ci]),
+        "gctf_set_proposal_params": (ci, [vp, fp, ctypes.c_longlong]),
+        "gctf_estimate_batch_learned": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf,
+                                             ctypes.POINTER(GctfSearchProgram), cf, cf, fp, fp]),
+        "gctf_net_param_count": (ctypes.c_longlong, []),
+        "gctf_net_create": (ci, [ci, ci, ctypes.POINTER(vp)]),
+        "gctf_net_destroy": (None, [vp]),
+        "gctf_net_error_string": (ctypes.c_char_p, [vp]),
+        "gctf_net_set_params": (ci, [vp, fp, ctypes.c_longlong]),
+        "gctf_net_get_params": (ci, [vp, fp, ctypes.c_longlong]),
+        "gctf_net_reset_optimizer": (ci, [vp]),
+        "gctf_net_forward": (ci, [vp, vp, ci, ci, ci, fp]),
+        "gctf_net_loss_grad": (ci, [vp, vp, ci, vp, ci, ci, fp, fp]),
+        "gctf_net_adam_step": (ci, [vp, cf, cf, cf, cf, cf]),
+        "gctf_net_synchronize": (ci, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -245,6 +283,28 @@ class Context:
         out = [r.as_dict() for r in res]
         return (out, cc, top, rs) if debug else out
 
+    def radial_batch(self, dev_ptr, n, nx, ny, pix, dev_out_ptr, ld, dtype=DTYPE_F32, stride=None):
+        """Spectrum stages + radial average into a device buffer [n][ld]."""
+        stride = nx * ny if stride is None else stride
+        self.check(self.L.gctf_radial_batch(self.h, ctypes.c_void_p(dev_ptr), dtype, n, nx, ny, stride, pix,
+                                            ctypes.c_void_p(dev_out_ptr), ld))
+
+    def set_proposal_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self.check(self.L.gctf_set_proposal_params(self.h, _np_ptr(p), p.size))
+
+    def estimate_batch_learned(self, dev_ptr, n, nx, ny, pix, program=None, conf_threshold=0.8,
+                               window=300.0, dtype=DTYPE_F32, stride=None, proposals=False):
+        stride = nx * ny if stride is None else stride
+        prog = None if program is None else ctypes.byref(GctfSearchProgram(**program))
+        res = (GctfResult * n)()
+        props = (GctfProposal * n)()
+        self.check(self.L.gctf_estimate_batch_learned(
+            self.h, ctypes.c_void_p(dev_ptr), dtype, n, nx, ny, stride, pix, prog, conf_threshold, window,
+            ctypes.cast(res, ctypes.c_void_p), ctypes.cast(props, ctypes.c_void_p)))
+        out = [r.as_dict() for r in res]
+        return (out, [p.as_dict() for p in props]) if proposals else out
+
     def tune(self, chunk=0, tile_splits=0, use_graph=0):
         self.check(self.L.gctf_tune(self.h, chunk, tile_splits, use_graph))
 
@@ -266,6 +326,73 @@ class Context:
 
     def synchronize(self):
         self.check(self.L.gctf_synchronize(self.h))
+
+
+def learned_program_default():
+    L = load()
+    p = GctfSearchProgram()
+    L.gctf_learned_program_default(ctypes.byref(p))
+    return p.as_dict()
+
+
+class Net:
+    """Trainer of the proposal network (gctf_net_* in the C-ABI)."""
+
+    def __init__(self, max_batch, device=0):
+        self.L = load()
+        self.h = ctypes.c_void_p()
+        rc = self.L.gctf_net_create(device, max_batch, ctypes.byref(self.h))
+        if rc != GCTF_OK:
+            raise GctfError(rc, "gctf_net_create failed")
+        self.n_params = int(self.L.gctf_net_param_count())
+
+    def close(self):
+        if self.h:
+            self.L.gctf_net_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != GCTF_OK:
+            raise GctfError(rc, self.L.gctf_net_error_string(self.h).decode())
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self.check(self.L.gctf_net_set_params(self.h, _np_ptr(p), p.size))
+
+    def get_params(self):
+        p = np.zeros(self.n_params, np.float32)
+        self.check(self.L.gctf_net_get_params(self.h, _np_ptr(p), p.size))
+        return p
+
+    def reset_optimizer(self):
+        self.check(self.L.gctf_net_reset_optimizer(self.h))
+
+    def forward(self, dev_radial_ptr, ld, n, bf16=True):
+        out = np.zeros((n, 4), np.float32)
+        self.check(self.L.gctf_net_forward(self.h, ctypes.c_void_p(dev_radial_ptr), ld, n, int(bf16),
+                                           _np_ptr(out)))
+        return out
+
+    def loss_grad(self, dev_radial_ptr, ld, dev_target_ptr, n, bf16=True, want_loss=True, want_grad=False):
+        loss = np.zeros(1, np.float64)
+        g = np.zeros(self.n_params, np.float32) if want_grad else None
+        self.check(self.L.gctf_net_loss_grad(self.h, ctypes.c_void_p(dev_radial_ptr), ld,
+                                             ctypes.c_void_p(dev_target_ptr), n, int(bf16),
+                                             _np_ptr(loss) if want_loss else None,
+                                             _np_ptr(g) if want_grad else None))
+        return (float(loss[0]) if want_loss else None), g
+
+    def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0):
+        self.check(self.L.gctf_net_adam_step(self.h, lr, beta1, beta2, eps, weight_decay))
+
+    def synchronize(self):
+        self.check(self.L.gctf_net_synchronize(self.h))
 
 
 class CFindCtf2D:

@@ -158,6 +158,81 @@ GCTF_API int gctf_profile(gctf_ctx* ctx, int enable);
 GCTF_API int gctf_profile_report(gctf_ctx* ctx, int max, char* names, double* total_ms,
 	int* count, int* nout);
 
+/* ---- learned-proposal path (no reference counterpart) -------------------
+ * A 1D convolutional network reads each micrograph's radial spectrum (the
+ * CFindCtf1D input, CFindCtf1D.cpp:113-117) and proposes its mean defocus
+ * with a confidence.  Confident proposals seed a narrowed 2D search (the
+ * CFindCtf2D::Refine hook, CFindCtf2D.cpp:55-68, with DoIt's astigmatism
+ * box, CFindDefocus2D.cpp:113-118, instead of Refine's 0.02 cap); the
+ * others, and narrowed searches that fail or end at the window edge, get
+ * the full plain search (the reference's score <= 0 fallback,
+ * CFindSeriesCtfs.cpp:144). */
+
+/* Candidate counts of a 2D search program.  The reference's (the plain
+ * path): 51 x 51 grid, 21 / 51 / 31 line candidates, 20 rounds, 20 A floor,
+ * no fine stage. */
+typedef struct {
+	int n_ast, n_ang;     /* astigmatism x azimuth grid of the first stage */
+	int n_mag, n_angl;    /* astigmatism-magnitude / azimuth line candidates */
+	int n_df;             /* mean-defocus line candidates */
+	int iters;            /* rounds (mIterate's iIterations) */
+	float df_minstep;     /* mean-defocus step floor, A */
+	int fine_n;           /* final mean-defocus stage: candidates (0: none) */
+	float fine_step;      /* its step, A */
+} gctf_search_program;
+
+typedef struct {
+	float df_mean;        /* proposed mean defocus, A */
+	float astig;          /* proposed df_max - df_min, A */
+	float azimuth_deg;    /* proposed azimuth (a radial spectrum carries none; informative only) */
+	float confidence;     /* in [0, 1] */
+	int fallback;         /* 1: the plain search produced this micrograph's result */
+} gctf_proposal;
+
+/* Defaults of the learned path's narrowed search program. */
+GCTF_API int gctf_learned_program_default(gctf_search_program* out);
+
+/* Spectrum stages as gctf_estimate_batch, then the radial average of every
+ * micrograph (CFindCtf1D.cpp:113-117): d_radial_out [n][ld] device floats,
+ * bins 0 .. min(ld, tile/2 + 1) - 1. */
+GCTF_API int gctf_radial_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny,
+	long long stride, float pix, float* d_radial_out, int ld);
+
+/* Install the proposal network's parameters (gctf_net_param_count() FP32
+ * values, host memory, the layout gctf_net_get_params returns). */
+GCTF_API int gctf_set_proposal_params(gctf_ctx* ctx, const float* host_params, long long count);
+
+/* The learned path for a batch of device-resident micrographs: spectrum
+ * stages, proposal network (BF16 MFMA), the narrowed search of `prog`
+ * (NULL: gctf_learned_program_default) in [df - window_A, df + window_A],
+ * fallback to the plain search below conf_threshold.  prop_out optional. */
+GCTF_API int gctf_estimate_batch_learned(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx,
+	int ny, long long stride, float pix, const gctf_search_program* prog, float conf_threshold,
+	float window_A, gctf_result* out, gctf_proposal* prop_out);
+
+/* ---- proposal-network trainer (no reference counterpart) ----------------
+ * Forward and backward passes on MFMA (bf16 = 1: BF16 operands, FP32
+ * accumulation and master parameters; bf16 = 0: FP32 operands, for gradient
+ * checks), AdamW on the FP32 parameters.  Device pointers are read on the
+ * trainer's own stream: synchronise the producer first. */
+typedef struct gctf_net gctf_net;
+GCTF_API long long gctf_net_param_count(void);
+GCTF_API int gctf_net_create(int device, int max_batch, gctf_net** out);
+GCTF_API void gctf_net_destroy(gctf_net* net);
+GCTF_API const char* gctf_net_error_string(const gctf_net* net);
+GCTF_API int gctf_net_set_params(gctf_net* net, const float* host, long long count);
+GCTF_API int gctf_net_get_params(gctf_net* net, float* host, long long count);
+GCTF_API int gctf_net_reset_optimizer(gctf_net* net);
+/* raw outputs [n][4]: ln(mean df / 1e4 A), astigmatism / 1e3 A, azimuth rad, confidence logit */
+GCTF_API int gctf_net_forward(gctf_net* net, const float* d_radial, int ld, int n, int bf16, float* host_out);
+/* d_target [n][4] device: mean df A, astigmatism A, azimuth deg, unused.
+ * Leaves the gradient for gctf_net_adam_step; loss / host_grad optional. */
+GCTF_API int gctf_net_loss_grad(gctf_net* net, const float* d_radial, int ld, const float* d_target, int n,
+	int bf16, double* loss, float* host_grad);
+GCTF_API int gctf_net_adam_step(gctf_net* net, float lr, float beta1, float beta2, float eps,
+	float weight_decay);
+GCTF_API int gctf_net_synchronize(gctf_net* net);
+
 #ifdef __cplusplus
 }
 #endif

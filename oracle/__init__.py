@@ -1,4 +1,4 @@
-"""ctypes binding of the CPU oracle (oracle/gctf_oracle.c).
+"""ctypes binding of the CPU oracle (oracle/gctf_oracle.c, oracle/net_oracle.c).
 
 Test infrastructure only: tests/, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg load this module as the checker.  The product package
@@ -56,6 +56,9 @@ def lib():
         "orc_ring_pixels": (_c_int, [_c_int, _c_int, _c_float, _c_float, _c_void]),
         "orc_cc2d": (ctypes.c_double, [_f64p, _f64p, _c_int, _c_int, _c_float, _c_float, _c_float]),
         "orc_spectral_cc": (_c_int, [_f64p, _f64p, _c_int, _c_int, _c_int, _c_void]),
+        "orc_net_param_count": (ctypes.c_long, []),
+        "orc_net_forward": (None, [_f64p, _f32p, _c_int, _c_int, _f64p]),
+        "orc_net_loss_grad": (ctypes.c_double, [_f64p, _f32p, _c_int, _f32p, _c_int, _c_void]),
         "orc_fc_create": (_c_void, []),
         "orc_fc_destroy": (None, [_c_void]),
         "orc_fc_setup": (None, [_c_void, _c_float, _c_float, _c_float, _c_float, _c_int, _c_int, _c_float]),
@@ -333,3 +336,27 @@ def grid_cc(spect, grid, pix, kv=300.0, cs=2.7, ac=0.07, ext_phase_deg=0.0):
     lib().orc_grid_cc(spect, spect.shape[1], spect.shape[0], kv, cs, ac, pix, ext_phase_deg, gf, gi,
                       grid["res_lo"], grid["res_hi"], out)
     return out
+
+
+# ---- proposal network (net_oracle.c) --------------------------------------
+def net_param_count():
+    return int(lib().orc_net_param_count())
+
+
+def net_forward(params, radial):
+    """FP64 forward pass: radial [n][>=256] float32 -> raw outputs [n][4]."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    r = np.ascontiguousarray(radial, dtype=np.float32)
+    out = np.zeros((r.shape[0], 4), np.float64)
+    lib().orc_net_forward(p, r, r.shape[1], r.shape[0], out)
+    return out
+
+
+def net_loss_grad(params, radial, target, grad=True):
+    """FP64 mean loss and its gradient over the parameters."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    r = np.ascontiguousarray(radial, dtype=np.float32)
+    t = np.ascontiguousarray(target, dtype=np.float32)
+    g = np.zeros(p.size, np.float64) if grad else None
+    loss = lib().orc_net_loss_grad(p, r, r.shape[1], t, r.shape[0], g.ctypes.data if grad else None)
+    return (float(loss), g) if grad else float(loss)
