@@ -163,6 +163,7 @@ struct gctf_ctx {
 	std::map<int, std::unique_ptr<TileTables>> tiles;
 	std::map<std::string, std::unique_ptr<SearchPlan>> splans;
 	std::map<std::string, GraphEntry> graphs;
+	std::map<std::string, std::pair<int, int>> tuned;  // shape -> (chunk, tile splits) from gctf_autotune
 
 	// single-image (reference-style) state
 	RescalePlan rp;
@@ -176,10 +177,12 @@ struct gctf_ctx {
 	int cap = 0;  // images the per-image buffers hold
 	DevBuf cbufA, cbufB, padc, bsums, rt, tpart;
 	DevBuf raw, hsum, spect, mom, q, radial, ring_spec, ring_ss, states, part2d, refine, shell_cc, fullsp,
-		moments1;
+		moments1, plans;
+	int kstride = 0;  // ring pixels reserved per image in ring_spec
 	DevBuf g_bt, g_fn, g_gt, g_cc, g_part, g_top, g_rs, g_rescored;
 	// learned path: proposal-network parameters, activations, outputs
 	DevBuf net_params, net_out, props;
+	DevBuf series;  // tilt-series controller state
 	bool have_net = false;
 	NetBuffers* netb = nullptr;
 	std::vector<SearchState> h_states;
@@ -524,6 +527,7 @@ int ensure_image_buffers(gctf_ctx* c, int n, int T, int ring_n)
 	CHK(ensure_buf(c, c->ring_ss, sizeof(double) * n));
 	CHK(ensure_buf(c, c->part2d, sizeof(double2) * (size_t)kMaxCands * kPixChunks * n));
 	CHK(ensure_buf(c, c->refine, sizeof(RefineArgs) * n));
+	CHK(ensure_buf(c, c->plans, sizeof(ImgPlan) * n));
 	CHK(ensure_buf(c, c->shell_cc, sizeof(float) * cx * n));
 	size_t old = c->states.bytes;
 	CHK(ensure_buf(c, c->states, sizeof(SearchState) * n));
@@ -706,63 +710,127 @@ int launch_post(gctf_ctx* c, const TileTables& tt, int n)
 	}
 }
 
-// Radial average + CFindCtf1D::Do1D / Refine1D for images [0, n).
-int launch_1d(gctf_ctx* c, const SearchPlan& sp, const TileTables& tt, int n, int mode, float init_df,
-	float range)
+// Per-image search plans: images [i0, i0 + m) get the plan of `sp`.
+__global__ void k_fill_plans(ImgPlan* __restrict__ dst, int m, ImgPlan v)
 {
-	hipStream_t s = c->stream;
-	const int T = sp.T, cx = T / 2 + 1;
-	PROF(c, "k_radial", k_radial<<<n, 256, 0, s>>>(c->spect.as<float>(), cx, T, tt.roff.as<int>(), tt.rcnt.as<int>(),
-		tt.ridx.as<int>(), c->radial.as<float>()));
-	Search1DParams p;
-	p.c = sp.cc; p.cmpx = cx; p.lo = sp.lo1d; p.hi = sp.hi1d;
-	p.mode = mode; p.init_df = init_df; p.range = range;
-	PROF(c, "k_search1d", k_search1d<<<n, 256, 0, s>>>(p, c->radial.as<float>(), tt.w1d.as<double>(), c->states.as<SearchState>()));
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < m) dst[i] = v;
+}
+
+int fill_plans(gctf_ctx* c, int i0, int m, const SearchPlan& sp)
+{
+	if (sp.ring.n > c->kstride) return fail(c, GCTF_ERR_STATE, "ring larger than the reserved stride");
+	ImgPlan v;
+	v.cc = sp.cc;
+	v.ring = sp.ring;
+	v.lo1d = sp.lo1d;
+	v.hi1d = sp.hi1d;
+	k_fill_plans<<<(m + 63) / 64, 64, 0, c->stream>>>(c->plans.as<ImgPlan>() + i0, m, v);
 	HIPCHK(c, hipGetLastError());
 	return GCTF_OK;
 }
 
-// The CFindDefocus2D::mIterate stage program + mCalcCtfRes for images [0, n).
-int launch_2d(gctf_ctx* c, const SearchPlan& sp, int n, int begin, float phase_range)
+// Radial average + CFindCtf1D::Do1D / Refine1D for images [i0, i0 + m).
+int launch_1d(gctf_ctx* c, const TileTables& tt, int i0, int m, int mode, float init_df, float range)
 {
 	hipStream_t s = c->stream;
-	const int T = sp.T, cx = T / 2 + 1;
-	SearchState* st = c->states.as<SearchState>();
-	double2* part = c->part2d.as<double2>();
-	const double* ss = c->ring_ss.as<double>();
-	PROF(c, "k_ring_gather", k_ring_gather<<<n, 1024, 0, s>>>(c->spect.as<float>(), T * cx, sp.ring, c->ring_spec.as<float>(),
-		c->ring_ss.as<double>()));
+	const int T = tt.T, cx = T / 2 + 1;
+	const float* spect = c->spect.as<float>() + (size_t)i0 * T * cx;
+	float* radial = c->radial.as<float>() + (size_t)i0 * cx;
+	PROF(c, "k_radial", k_radial<<<dim3((cx + 63) / 64, m), 64, 0, s>>>(spect, cx, T, tt.roff.as<int>(),
+		tt.rcnt.as<int>(), tt.ridx.as<int>(), radial));
+	Search1DParams p{};
+	p.cmpx = cx;
+	p.mode = mode; p.init_df = init_df; p.range = range;
+	PROF(c, "k_search1d", k_search1d<<<m, 256, 0, s>>>(p, c->plans.as<ImgPlan>() + i0, radial,
+		tt.w1d.as<double>(), c->states.as<SearchState>() + i0));
+	HIPCHK(c, hipGetLastError());
+	return GCTF_OK;
+}
+
+// Device pointers of the 2D search for images [i0, i0 + m).
+struct Search2DView {
+	const float* spect;
+	const ImgPlan* plans;
+	float* ring_spec;
+	double* ring_ss;
+	SearchState* st;
+	double2* part;
+	RefineArgs* refine;
+	float* shell_cc;
+	int T, cx, kstride;
+};
+
+Search2DView view_2d(gctf_ctx* c, int T, int i0)
+{
+	Search2DView v;
+	v.T = T;
+	v.cx = T / 2 + 1;
+	v.kstride = c->kstride;
+	v.spect = c->spect.as<float>() + (size_t)i0 * T * v.cx;
+	v.plans = c->plans.as<ImgPlan>() + i0;
+	v.ring_spec = c->ring_spec.as<float>() + (size_t)i0 * c->kstride;
+	v.ring_ss = c->ring_ss.as<double>() + i0;
+	v.st = c->states.as<SearchState>() + i0;
+	v.part = c->part2d.as<double2>() + (size_t)i0 * kMaxCands * kPixChunks;
+	v.refine = c->refine.as<RefineArgs>() + i0;
+	v.shell_cc = c->shell_cc.as<float>() + (size_t)i0 * v.cx;
+	return v;
+}
+
+// A 2D search program: `begin`, INIT, the astigmatism grid, `iters` - 1
+// rounds of (ASTMAG, ASTANG, DFMEAN, PHASE), the optional fine defocus
+// stage, then mCalcCtfRes.  The plain program is CFindDefocus2D::mIterate
+// (CFindDefocus2D.cpp:153-176) with the reference's counts.
+int run_2d_program(gctf_ctx* c, const Search2DView& v, int m, int begin, const SearchProgram& pg,
+	float phase_range)
+{
+	hipStream_t s = c->stream;
+	PROF(c, "k_ring_gather", k_ring_gather<<<m, 1024, 0, s>>>(v.spect, v.T * v.cx, v.plans, v.kstride,
+		v.ring_spec, v.ring_ss));
 	constexpr int CPB = 8;
 	auto eval = [&](int maxc) {
-		dim3 grid((maxc + CPB - 1) / CPB, kPixChunks, n);
-		PROF(c, "k_cc2d_eval", k_cc2d_eval<CPB><<<grid, 256, 0, s>>>(sp.cc, sp.ring, c->ring_spec.as<float>(), st, part));
+		dim3 grid((maxc + CPB - 1) / CPB, kPixChunks, m);
+		PROF(c, "k_cc2d_eval", k_cc2d_eval<CPB><<<grid, 256, 0, s>>>(v.plans, v.ring_spec, v.kstride, v.st,
+			v.part));
 	};
 	auto step = [&](int next) {
-		PROF(c, "k_s2d_step", k_s2d_step<<<n, 256, 0, s>>>(st, part, ss, next, c->refine.as<RefineArgs>(),
-			plain_program()));
+		PROF(c, "k_s2d_step", k_s2d_step<<<m, 256, 0, s>>>(v.st, v.part, v.ring_ss, next, v.refine, pg));
 	};
 	int maxph = 61;
 	if (phase_range > 61.0f) maxph = std::min(kMaxCands, (int)phase_range + 3);
+	const int last = pg.fine_n >= 2 ? NX_FINE : NX_FINISH;
 	step(begin);
 	eval(1);
 	step(NX_ASTIG);
-	eval(kMaxCands);
+	eval(std::min(kMaxCands, pg.n_ast * pg.n_ang));
 	step(NX_ASTMAG);
-	for (int i = 1; i < 20; ++i) {
-		eval(21);
+	for (int i = 1; i < pg.iters; ++i) {
+		// line stages hold at most n + 1 candidates (odd counts, prep_line / prep_dfmean)
+		eval(pg.n_mag + 1);
 		step(NX_ASTANG);
-		eval(51);
+		eval(pg.n_angl + 1);
 		step(NX_DFMEAN);
-		eval(31);
+		eval(pg.n_df + 1);
 		step(NX_PHASE);
 		if (phase_range > 0.0f) eval(maxph);
-		step(i < 19 ? NX_ASTMAG : NX_FINISH);
+		step(i < pg.iters - 1 ? NX_ASTMAG : last);
 	}
-	PROF(c, "k_spectral_cc", k_spectral_cc<<<dim3(cx, n), 256, 0, s>>>(sp.cc, cx, T, 10, c->spect.as<float>(), st,
-		c->shell_cc.as<float>()));
-	PROF(c, "k_ctf_res", k_ctf_res<<<(n + 63) / 64, 64, 0, s>>>(c->shell_cc.as<float>(), cx, T, sp.pix, st, n));
+	if (last == NX_FINE) {
+		eval(pg.fine_n + 1);
+		step(NX_FINISH);
+	}
+	PROF(c, "k_spectral_cc", k_spectral_cc<<<dim3(v.cx, m), 256, 0, s>>>(v.plans, v.cx, v.T, 10, v.spect, v.st,
+		v.shell_cc));
+	PROF(c, "k_ctf_res", k_ctf_res<<<(m + 63) / 64, 64, 0, s>>>(v.shell_cc, v.cx, v.T, v.plans, v.st, m));
 	HIPCHK(c, hipGetLastError());
 	return GCTF_OK;
+}
+
+// CFindDefocus2D::DoIt / Refine (plain program) for images [i0, i0 + m).
+int launch_2d(gctf_ctx* c, int T, int i0, int m, int begin, float phase_range)
+{
+	return run_2d_program(c, view_2d(c, T, i0), m, begin, plain_program(), phase_range);
 }
 
 // ---- learned path --------------------------------------------------------
@@ -823,54 +891,6 @@ __global__ void k_clear_skip(SearchState* __restrict__ st, int n)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) st[i].skip = 0;
-}
-
-// The learned path's 2D program: the mIterate stage sequence with the
-// counts of `pg`, in the proposal's window, then the fine defocus stage.
-int launch_2d_learned(gctf_ctx* c, const SearchPlan& sp, int n, const SearchProgram& pg, float phase_range)
-{
-	hipStream_t s = c->stream;
-	const int T = sp.T, cx = T / 2 + 1;
-	SearchState* st = c->states.as<SearchState>();
-	double2* part = c->part2d.as<double2>();
-	const double* ss = c->ring_ss.as<double>();
-	PROF(c, "k_ring_gather", k_ring_gather<<<n, 1024, 0, s>>>(c->spect.as<float>(), T * cx, sp.ring,
-		c->ring_spec.as<float>(), c->ring_ss.as<double>()));
-	constexpr int CPB = 8;
-	auto eval = [&](int maxc) {
-		dim3 grid((maxc + CPB - 1) / CPB, kPixChunks, n);
-		PROF(c, "k_cc2d_eval", k_cc2d_eval<CPB><<<grid, 256, 0, s>>>(sp.cc, sp.ring, c->ring_spec.as<float>(), st, part));
-	};
-	auto step = [&](int next) {
-		PROF(c, "k_s2d_step", k_s2d_step<<<n, 256, 0, s>>>(st, part, ss, next, c->refine.as<RefineArgs>(), pg));
-	};
-	int maxph = 61;
-	if (phase_range > 61.0f) maxph = std::min(kMaxCands, (int)phase_range + 3);
-	const int last = pg.fine_n >= 2 ? NX_FINE : NX_FINISH;
-	step(NX_BEGIN_PROPOSAL);
-	eval(1);
-	step(NX_ASTIG);
-	eval(pg.n_ast * pg.n_ang);
-	step(NX_ASTMAG);
-	for (int i = 1; i < pg.iters; ++i) {
-		eval(pg.n_mag + 1);
-		step(NX_ASTANG);
-		eval(pg.n_angl + 1);
-		step(NX_DFMEAN);
-		eval(pg.n_df + 1);
-		step(NX_PHASE);
-		if (phase_range > 0.0f) eval(maxph);
-		step(i < pg.iters - 1 ? NX_ASTMAG : last);
-	}
-	if (last == NX_FINE) {
-		eval(pg.fine_n + 1);
-		step(NX_FINISH);
-	}
-	PROF(c, "k_spectral_cc", k_spectral_cc<<<dim3(cx, n), 256, 0, s>>>(sp.cc, cx, T, 10, c->spect.as<float>(), st,
-		c->shell_cc.as<float>()));
-	PROF(c, "k_ctf_res", k_ctf_res<<<(n + 63) / 64, 64, 0, s>>>(c->shell_cc.as<float>(), cx, T, sp.pix, st, n));
-	HIPCHK(c, hipGetLastError());
-	return GCTF_OK;
 }
 
 __global__ void k_state_init(SearchState* st, int n, float p0, float range)
@@ -965,6 +985,8 @@ int current_plan(gctf_ctx* c, SearchPlan** sp, TileTables** tt)
 	CHK(get_search_plan(c, c->s_kv, c->s_cs, c->s_ac, c->s_pix, c->prm.tile_size, sp));
 	CHK(get_tile_tables(c, c->prm.tile_size, tt));
 	CHK(ensure_image_buffers(c, 1, c->prm.tile_size, (*sp)->ring.n));
+	c->kstride = (*sp)->ring.n;
+	CHK(fill_plans(c, 0, 1, **sp));
 	return GCTF_OK;
 }
 
@@ -1218,7 +1240,7 @@ int gctf_do1d(gctf_ctx* ctx, gctf_result* out)
 	SearchPlan* sp;
 	TileTables* tt;
 	CHK(current_plan(ctx, &sp, &tt));
-	CHK(launch_1d(ctx, *sp, *tt, 1, 0, 0.f, 0.f));
+	CHK(launch_1d(ctx, *tt, 0, 1, 0, 0.f, 0.f));
 	return single_result(ctx, out);
 }
 
@@ -1230,7 +1252,7 @@ int gctf_refine1d(gctf_ctx* ctx, float init_df, float df_range, gctf_result* out
 	SearchPlan* sp;
 	TileTables* tt;
 	CHK(current_plan(ctx, &sp, &tt));
-	CHK(launch_1d(ctx, *sp, *tt, 1, 1, init_df, df_range));
+	CHK(launch_1d(ctx, *tt, 0, 1, 1, init_df, df_range));
 	return single_result(ctx, out);
 }
 
@@ -1246,8 +1268,8 @@ int gctf_do2d(gctf_ctx* ctx, gctf_result* out)
 	HIPCHK(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->states.p, sizeof(SearchState), hipMemcpyDeviceToHost, ctx->stream));
 	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
 	const float prange = ctx->h_pinned[0].phase_range;
-	CHK(launch_1d(ctx, *sp, *tt, 1, 0, 0.f, 0.f));
-	CHK(launch_2d(ctx, *sp, 1, NX_BEGIN_DO2D, prange));
+	CHK(launch_1d(ctx, *tt, 0, 1, 0, 0.f, 0.f));
+	CHK(launch_2d(ctx, ctx->prm.tile_size, 0, 1, NX_BEGIN_DO2D, prange));
 	return single_result(ctx, out);
 }
 
@@ -1267,7 +1289,7 @@ int gctf_refine(gctf_ctx* ctx, const float* df_mean, const float* ast_ratio, con
 		r.ang[i] = ast_angle_deg[i]; r.phase[i] = ext_phase_deg[i];
 	}
 	HIPCHK(ctx, hipMemcpyAsync(ctx->refine.p, &r, sizeof r, hipMemcpyHostToDevice, ctx->stream));
-	CHK(launch_2d(ctx, *sp, 1, NX_BEGIN_REFINE, ext_phase_deg[1]));
+	CHK(launch_2d(ctx, ctx->prm.tile_size, 0, 1, NX_BEGIN_REFINE, ext_phase_deg[1]));
 	return single_result(ctx, out);
 }
 
@@ -1403,7 +1425,7 @@ int launch_grid(gctf_ctx* c, const BatchJob& j, int n)
 	const SearchPlan& r = *j.gring;
 	const int T = r.T, cx = T / 2 + 1;
 	const int K = r.ring.n, kpad = (K + kGemmKs - 1) / kGemmKs * kGemmKs;
-	PROF(c, "k_ring_gather", k_ring_gather<<<n, 1024, 0, s>>>(c->spect.as<float>(), T * cx, r.ring,
+	PROF(c, "k_ring_gather", k_ring_gather_uniform<<<n, 1024, 0, s>>>(c->spect.as<float>(), T * cx, r.ring,
 		c->ring_spec.as<float>(), c->ring_ss.as<double>()));
 	PROF(c, "k_grid_tables", k_grid_tables<<<(kpad + 255) / 256, 256, 0, s>>>(r.cc, r.ring,
 		c->prm.ext_phase_deg, kpad, c->g_gt.as<float>()));
@@ -1441,7 +1463,7 @@ int record_batch(gctf_ctx* c, const RescalePlan& rp, SearchPlan& sp, TileTables&
 	if (job.mode == 2 || job.mode == 3) {
 		const int T = sp.T, cx = T / 2 + 1;
 		hipStream_t s = c->stream;
-		PROF(c, "k_radial", k_radial<<<n, 256, 0, s>>>(c->spect.as<float>(), cx, T, tt.roff.as<int>(),
+		PROF(c, "k_radial", k_radial<<<dim3((cx + 63) / 64, n), 64, 0, s>>>(c->spect.as<float>(), cx, T, tt.roff.as<int>(),
 			tt.rcnt.as<int>(), tt.ridx.as<int>(), c->radial.as<float>()));
 		if (job.mode == 3) {
 			HIPCHK(c, hipMemcpy2DAsync(job.radial_out, sizeof(float) * job.radial_ld, c->radial.p,
@@ -1452,13 +1474,57 @@ int record_batch(gctf_ctx* c, const RescalePlan& rp, SearchPlan& sp, TileTables&
 			c->net_out.as<float>(), s));
 		PROF(c, "k_proposal", k_proposal<<<(n + 63) / 64, 64, 0, s>>>(c->net_out.as<float>(), n, job.window,
 			c->props.as<float>(), c->refine.as<RefineArgs>(), c->states.as<SearchState>()));
-		CHK(launch_2d_learned(c, sp, n, job.pg, std::max(c->prm.ext_phase_range_deg, 0.0f)));
+		CHK(fill_plans(c, 0, n, sp));
+		CHK(run_2d_program(c, view_2d(c, T, 0), n, NX_BEGIN_PROPOSAL, job.pg,
+			std::max(c->prm.ext_phase_range_deg, 0.0f)));
 		PROF(c, "k_learned_check", k_learned_check<<<(n + 63) / 64, 64, 0, s>>>(c->states.as<SearchState>(), n,
 			job.conf_thr));
 		return GCTF_OK;
 	}
-	CHK(launch_1d(c, sp, tt, n, 0, 0.f, 0.f));
-	CHK(launch_2d(c, sp, n, NX_BEGIN_DO2D, std::max(c->prm.ext_phase_range_deg, 0.0f)));
+	CHK(fill_plans(c, 0, n, sp));
+	CHK(launch_1d(c, tt, 0, n, 0, 0.f, 0.f));
+	CHK(launch_2d(c, sp.T, 0, n, NX_BEGIN_DO2D, std::max(c->prm.ext_phase_range_deg, 0.0f)));
+	return GCTF_OK;
+}
+
+std::string shape_key(int nx, int ny, int dtype, float pix)
+{
+	char k[96];
+	snprintf(k, sizeof k, "%d/%d/%d/%a", nx, ny, dtype, pix);
+	return k;
+}
+
+// Chunk and tile-split counts chosen by gctf_autotune for this shape.
+void apply_tuning(gctf_ctx* ctx, int nx, int ny, int dtype, float pix)
+{
+	auto it = ctx->tuned.find(shape_key(nx, ny, dtype, pix));
+	if (it == ctx->tuned.end()) return;
+	ctx->chunk = it->second.first;
+	ctx->tile_splits = it->second.second;
+}
+
+// Record the launches of `record` into an executable graph cached under
+// `key` (re-captured after any device buffer moved) and launch it; without
+// graphs (or while profiling) the launches go straight to the stream.
+template <class F>
+int run_graph(gctf_ctx* ctx, const std::string& key, F&& record)
+{
+	if (!ctx->use_graph || ctx->prof) return record();
+	GraphEntry& ge = ctx->graphs[key];
+	if (!ge.exec || ge.gen != ctx->gen) {
+		if (ge.exec) { (void)hipGraphExecDestroy(ge.exec); ge.exec = nullptr; }
+		hipGraph_t graph = nullptr;
+		HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+		int r = record();
+		hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+		if (r != GCTF_OK) { if (graph) (void)hipGraphDestroy(graph); return r; }
+		HIPCHK(ctx, e);
+		hipError_t ei = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+		(void)hipGraphDestroy(graph);
+		HIPCHK(ctx, ei);
+		ge.gen = ctx->gen;
+	}
+	HIPCHK(ctx, hipGraphLaunch(ge.exec, ctx->stream));
 	return GCTF_OK;
 }
 
@@ -1471,6 +1537,7 @@ int run_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int
 	if (dtype != GCTF_DTYPE_F32 && dtype != GCTF_DTYPE_F16) return fail(ctx, GCTF_ERR_ARG, "dtype");
 	if (stride < (long long)nx * ny) return fail(ctx, GCTF_ERR_ARG, "stride smaller than one image");
 	HIPCHK(ctx, hipSetDevice(ctx->device));
+	apply_tuning(ctx, nx, ny, dtype, pix);
 	RescalePlan rp;
 	CHK(make_rescale_plan(ctx, nx, ny, pix, rp));
 	const int T = ctx->prm.tile_size;
@@ -1482,6 +1549,7 @@ int run_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int
 	CHK(get_search_plan(ctx, ctx->prm.kv, ctx->prm.cs_mm, ctx->prm.amp_contrast, rp.pixn, T, &sp));
 	CHK(get_tile_tables(ctx, T, &tt));
 	int ring_n = sp->ring.n;
+	ctx->kstride = sp->ring.n;
 	if (job.mode == 1) ring_n = std::max(ring_n, job.gring->ring.n);
 	const int C = std::max(1, std::min(ctx->chunk, n));
 	CHK(ensure_image_buffers(ctx, n, T, ring_n));
@@ -1501,28 +1569,10 @@ int run_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int
 		CHK(ensure_buf(ctx, ctx->net_out, sizeof(float) * kNetOut * n));
 		CHK(ensure_buf(ctx, ctx->props, sizeof(float) * 4 * n));
 	}
-	if (!ctx->use_graph || ctx->prof) {
-		CHK(record_batch(ctx, rp, *sp, *tt, dev_imgs, dtype, n, stride, job));
-		return read_results(ctx, n, out);
-	}
 	char key[320];
 	snprintf(key, sizeof key, "%p/%d/%d/%d/%d/%lld/%a/%d/%d/%s", dev_imgs, dtype, n, nx, ny, stride, pix, C,
 		ctx->tile_splits, job_key);
-	GraphEntry& ge = ctx->graphs[key];
-	if (!ge.exec || ge.gen != ctx->gen) {
-		if (ge.exec) { (void)hipGraphExecDestroy(ge.exec); ge.exec = nullptr; }
-		hipGraph_t graph = nullptr;
-		HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-		int r = record_batch(ctx, rp, *sp, *tt, dev_imgs, dtype, n, stride, job);
-		hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
-		if (r != GCTF_OK) { if (graph) (void)hipGraphDestroy(graph); return r; }
-		HIPCHK(ctx, e);
-		hipError_t ei = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
-		(void)hipGraphDestroy(graph);
-		HIPCHK(ctx, ei);
-		ge.gen = ctx->gen;
-	}
-	HIPCHK(ctx, hipGraphLaunch(ge.exec, ctx->stream));
+	CHK(run_graph(ctx, key, [&]() { return record_batch(ctx, rp, *sp, *tt, dev_imgs, dtype, n, stride, job); }));
 	return read_results(ctx, n, out);
 }
 
@@ -1668,8 +1718,9 @@ int gctf_estimate_batch_learned(gctf_ctx* ctx, const void* dev_imgs, int dtype, 
 		const float range = std::max(ctx->prm.ext_phase_range_deg, 0.0f);
 		SearchState* st = ctx->states.as<SearchState>();
 		k_fallback_prepare<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n, ctx->prm.ext_phase_deg, range);
-		CHK(launch_1d(ctx, *sp, *tt, n, 0, 0.f, 0.f));
-		CHK(launch_2d(ctx, *sp, n, NX_BEGIN_DO2D, range));
+		CHK(fill_plans(ctx, 0, n, *sp));
+		CHK(launch_1d(ctx, *tt, 0, n, 0, 0.f, 0.f));
+		CHK(launch_2d(ctx, ctx->prm.tile_size, 0, n, NX_BEGIN_DO2D, range));
 		k_clear_skip<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n);
 		HIPCHK(ctx, hipGetLastError());
 		CHK(read_results(ctx, n, out));
@@ -1698,5 +1749,286 @@ int gctf_learned_program_default(gctf_search_program* out)
 	out->df_minstep = 20.0f;
 	out->fine_n = 9;
 	out->fine_step = 5.0f;
+	return GCTF_OK;
+}
+
+// ---- tilt series (config: one series, same shape) --------------------------
+namespace {
+
+struct SeriesCtl {
+	int ref;   // CFindSeriesCtfs's reference package: best score so far
+};
+
+__global__ void k_series_set_ref(SeriesCtl* ctl, int k) { ctl->ref = k; }
+
+// CFindSeriesCtfs::mProcessRefine arguments (CFindSeriesCtfs.cpp:168-203)
+// from the reference package's results.
+__global__ void k_series_refine_prep(const SearchState* __restrict__ st, RefineArgs* __restrict__ refine,
+	const SeriesCtl* __restrict__ ctl, int k, float pix_in, int tomo, int phase_on)
+{
+	const SearchState& R = st[ctl->ref];
+	RefineArgs r;
+	const float range = 5000.0f * pix_in * pix_in;
+	r.dfmean[0] = 0.5f * (R.df_min + R.df_max);
+	r.dfmean[1] = fmaxf(r.dfmean[0] * 0.5f, range);
+	// CFindCtfHelp::CalcAstRatio (CFindCtfHelp.cpp:5-11)
+	const float diff = (R.df_max - R.df_min) * 0.5f, mean = (R.df_max + R.df_min) * 0.5f;
+	r.ast[0] = mean <= 0.0f ? 0.0f : diff / mean;
+	r.ang[0] = R.ast_ang;
+	r.phase[0] = R.ext_phase;
+	r.ast[1] = 0.0f; r.ang[1] = 0.0f; r.phase[1] = 0.0f;
+	if (!tomo) {
+		r.ast[1] = 0.04f;
+		r.ang[1] = 30.0f;
+		if (phase_on) r.phase[1] = 40.0f;
+	}
+	refine[k] = r;
+}
+
+// After a refine: the better score becomes the reference (:200-202); a
+// score <= 0 sends the package to a full search (:144), which then runs
+// with CFindCtfBase::SetPhase's values; otherwise the full search sits out.
+__global__ void k_series_after_refine(SearchState* __restrict__ st, SeriesCtl* __restrict__ ctl, int k, float p0,
+	float range)
+{
+	SearchState& s = st[k];
+	if (st[ctl->ref].score < s.score) ctl->ref = k;
+	if (s.score <= 0.0f) {
+		s.skip = 0;
+		s.fallback = 1;
+		s.ext_phase = p0;
+		s.phase_range = range;
+	} else {
+		s.skip = 1;
+		s.fallback = 0;
+	}
+}
+
+// mProcessFull makes its package the reference (:164).
+__global__ void k_series_after_full(SearchState* __restrict__ st, SeriesCtl* __restrict__ ctl, int k)
+{
+	SearchState& s = st[k];
+	if (!s.skip) ctl->ref = k;
+	s.skip = 0;
+}
+
+// CInputFolder::FindZeroTilt (CInputFolder.cpp:49-68)
+int zero_tilt(const float* tilts, int n)
+{
+	if (n < 2) return 0;
+	if (fabsf(tilts[0] - tilts[1]) < 0.0001f) return n / 2;
+	int z = -1;
+	float m = 10000.0f;
+	for (int i = 0; i < n; ++i) {
+		const float t = fabsf(tilts[i]);
+		if (t >= m) continue;
+		m = t;
+		z = i;
+	}
+	return z < 0 ? n / 2 : z;
+}
+
+}  // namespace
+
+int gctf_estimate_series(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny,
+	long long stride, float pix, const float* tilts_deg, int tomo, gctf_result* out)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	if (!dev_imgs || !out || !tilts_deg || n < 1) return fail(ctx, GCTF_ERR_ARG, "null input/output or n < 1");
+	if (dtype != GCTF_DTYPE_F32 && dtype != GCTF_DTYPE_F16) return fail(ctx, GCTF_ERR_ARG, "dtype");
+	if (stride < (long long)nx * ny) return fail(ctx, GCTF_ERR_ARG, "stride smaller than one image");
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	apply_tuning(ctx, nx, ny, dtype, pix);
+	RescalePlan rp;
+	CHK(make_rescale_plan(ctx, nx, ny, pix, rp));
+	const int T = ctx->prm.tile_size;
+	if (rp.nnx < T || rp.nny < T) return fail(ctx, GCTF_ERR_ARG, "image smaller than one tile");
+	const TileGrid g = make_grid(rp.nnx, rp.nny, T);
+	if (g.ntx < 1 || g.nty < 1) return fail(ctx, GCTF_ERR_ARG, "no tile fits the image");
+	SearchPlan* sp;
+	TileTables* tt;
+	CHK(get_search_plan(ctx, ctx->prm.kv, ctx->prm.cs_mm, ctx->prm.amp_contrast, rp.pixn, T, &sp));
+	CHK(get_tile_tables(ctx, T, &tt));
+	const int C = std::max(1, std::min(ctx->chunk, n));
+	CHK(ensure_image_buffers(ctx, n, T, sp->ring.n));
+	ctx->kstride = sp->ring.n;
+	CHK(ensure_spectrum_buffers(ctx, g, T, C));
+	if (rp.active) {
+		CHK(ensure_rescale_buffers(ctx, rp, C));
+		CHK(ensure_buf(ctx, ctx->padc, sizeof(float) * (size_t)rp.pitch * rp.nny * C));
+	}
+	CHK(ensure_buf(ctx, ctx->series, sizeof(SeriesCtl)));
+	// CLoadImages.cpp:71-91: the zero-tilt package first, the others in order
+	std::vector<int> order;
+	const int z = tomo ? zero_tilt(tilts_deg, n) : 0;
+	order.push_back(z);
+	for (int i = 0; i < n; ++i)
+		if (i != z) order.push_back(i);
+	const float p0 = ctx->prm.ext_phase_deg, range = std::max(ctx->prm.ext_phase_range_deg, 0.0f);
+	const int phase_on = ctx->prm.ext_phase_range_deg > 0.0f ? 1 : 0;
+	const float refine_phase = (!tomo && phase_on) ? 40.0f : 0.0f;
+	SeriesCtl* ctl = ctx->series.as<SeriesCtl>();
+	SearchState* st = ctx->states.as<SearchState>();
+	auto record = [&]() -> int {
+		CHK(record_spectra(ctx, rp, *tt, dev_imgs, dtype, n, stride));
+		CHK(fill_plans(ctx, 0, n, *sp));
+		const int k0 = order[0];
+		// mProcessFull for the first package (:155-166)
+		CHK(launch_1d(ctx, *tt, k0, 1, 0, 0.f, 0.f));
+		CHK(launch_2d(ctx, T, k0, 1, NX_BEGIN_DO2D, range));
+		k_series_set_ref<<<1, 1, 0, ctx->stream>>>(ctl, k0);
+		for (int j = 1; j < n; ++j) {
+			const int k = order[j];
+			k_series_refine_prep<<<1, 1, 0, ctx->stream>>>(st, ctx->refine.as<RefineArgs>(), ctl, k, pix, tomo,
+				phase_on);
+			CHK(launch_2d(ctx, T, k, 1, NX_BEGIN_REFINE, refine_phase));
+			k_series_after_refine<<<1, 1, 0, ctx->stream>>>(st, ctl, k, p0, range);
+			CHK(launch_1d(ctx, *tt, k, 1, 0, 0.f, 0.f));
+			CHK(launch_2d(ctx, T, k, 1, NX_BEGIN_DO2D, range));
+			k_series_after_full<<<1, 1, 0, ctx->stream>>>(st, ctl, k);
+		}
+		HIPCHK(ctx, hipGetLastError());
+		return GCTF_OK;
+	};
+	char key[192];
+	snprintf(key, sizeof key, "series/%p/%d/%d/%d/%d/%lld/%a/%d/%d/%d/%d", dev_imgs, dtype, n, nx, ny, stride, pix,
+		C, ctx->tile_splits, tomo, z);
+	CHK(run_graph(ctx, key, record));
+	return read_results(ctx, n, out);
+}
+
+// ---- uneven batch ------------------------------------------------------------
+int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result* out)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	if (!imgs || !out || n < 1) return fail(ctx, GCTF_ERR_ARG, "null input/output or n < 1");
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	const int T = ctx->prm.tile_size;
+	struct Item {
+		RescalePlan rp;
+		TileGrid g;
+		SearchPlan* sp;
+	};
+	std::vector<Item> it((size_t)n);
+	TileTables* tt;
+	CHK(get_tile_tables(ctx, T, &tt));
+	int kmax = 0;
+	std::string key = "list";
+	for (int i = 0; i < n; ++i) {
+		const gctf_image& im = imgs[i];
+		if (!im.data) return fail(ctx, GCTF_ERR_ARG, "null micrograph");
+		if (im.dtype != GCTF_DTYPE_F32 && im.dtype != GCTF_DTYPE_F16) return fail(ctx, GCTF_ERR_ARG, "dtype");
+		CHK(make_rescale_plan(ctx, im.nx, im.ny, im.pix, it[i].rp));
+		const RescalePlan& rp = it[i].rp;
+		if (rp.nnx < T || rp.nny < T) return fail(ctx, GCTF_ERR_ARG, "image smaller than one tile");
+		it[i].g = make_grid(rp.nnx, rp.nny, T);
+		if (it[i].g.ntx < 1 || it[i].g.nty < 1) return fail(ctx, GCTF_ERR_ARG, "no tile fits the image");
+		CHK(get_search_plan(ctx, ctx->prm.kv, ctx->prm.cs_mm, ctx->prm.amp_contrast, rp.pixn, T, &it[i].sp));
+		kmax = std::max(kmax, it[i].sp->ring.n);
+		char k[128];
+		snprintf(k, sizeof k, "/%p:%d:%d:%d:%a", im.data, im.dtype, im.nx, im.ny, im.pix);
+		key += k;
+	}
+	CHK(ensure_image_buffers(ctx, n, T, kmax));
+	ctx->kstride = kmax;
+	for (int i = 0; i < n; ++i) {
+		CHK(ensure_spectrum_buffers(ctx, it[i].g, T, 1));
+		if (it[i].rp.active) {
+			CHK(ensure_rescale_buffers(ctx, it[i].rp, 1));
+			CHK(ensure_buf(ctx, ctx->padc, sizeof(float) * (size_t)it[i].rp.pitch * it[i].rp.nny));
+		}
+	}
+	char tail[64];
+	snprintf(tail, sizeof tail, "/%d", ctx->tile_splits);
+	key += tail;
+	auto record = [&]() -> int {
+		k_state_init<<<(n + 63) / 64, 64, 0, ctx->stream>>>(ctx->states.as<SearchState>(), n,
+			ctx->prm.ext_phase_deg, std::max(ctx->prm.ext_phase_range_deg, 0.0f));
+		for (int i = 0; i < n; ++i) {
+			const gctf_image& im = imgs[i];
+			const RescalePlan& rp = it[i].rp;
+			const long long st = (long long)im.nx * im.ny;
+			if (rp.active) {
+				float* pad = ctx->padc.as<float>();
+				const long long ps = (long long)rp.pitch * rp.nny;
+				CHK(launch_rescale(ctx, rp, im.data, im.dtype, st, 1, pad, ps));
+				CHK(launch_spectrum<float>(ctx, it[i].g, *tt, pad, rp.pitch, ps, 1, i));
+			} else if (im.dtype == GCTF_DTYPE_F16) {
+				CHK(launch_spectrum<_Float16>(ctx, it[i].g, *tt, (const _Float16*)im.data, im.nx, st, 1, i));
+			} else {
+				CHK(launch_spectrum<float>(ctx, it[i].g, *tt, (const float*)im.data, im.nx, st, 1, i));
+			}
+		}
+		CHK(launch_post(ctx, *tt, n));
+		for (int i = 0; i < n; ++i) CHK(fill_plans(ctx, i, 1, *it[i].sp));
+		CHK(launch_1d(ctx, *tt, 0, n, 0, 0.f, 0.f));
+		CHK(launch_2d(ctx, T, 0, n, NX_BEGIN_DO2D, std::max(ctx->prm.ext_phase_range_deg, 0.0f)));
+		return GCTF_OK;
+	};
+	CHK(run_graph(ctx, key, record));
+	return read_results(ctx, n, out);
+}
+
+// ---- autotuner ---------------------------------------------------------------
+int gctf_autotune(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny, long long stride,
+	float pix, int* best_chunk, int* best_splits, float* best_ms)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	if (!dev_imgs || n < 1) return fail(ctx, GCTF_ERR_ARG, "null input or n < 1");
+	if (stride < (long long)nx * ny) return fail(ctx, GCTF_ERR_ARG, "stride smaller than one image");
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	RescalePlan rp;
+	CHK(make_rescale_plan(ctx, nx, ny, pix, rp));
+	const int T = ctx->prm.tile_size;
+	if (rp.nnx < T || rp.nny < T) return fail(ctx, GCTF_ERR_ARG, "image smaller than one tile");
+	const TileGrid g = make_grid(rp.nnx, rp.nny, T);
+	TileTables* tt;
+	SearchPlan* sp;
+	CHK(get_tile_tables(ctx, T, &tt));
+	CHK(get_search_plan(ctx, ctx->prm.kv, ctx->prm.cs_mm, ctx->prm.amp_contrast, rp.pixn, T, &sp));
+	CHK(ensure_image_buffers(ctx, n, T, sp->ring.n));
+	const int chunks[] = {1, 2, 4, 8, 16};
+	const int splits[] = {4, 8, 16};
+	const int save_c = ctx->chunk, save_s = ctx->tile_splits;
+	hipEvent_t a, b;
+	HIPCHK(ctx, hipEventCreate(&a));
+	HIPCHK(ctx, hipEventCreate(&b));
+	float best = 1e30f;
+	int bc = save_c, bs = save_s;
+	int rc = GCTF_OK;
+	for (int ci : chunks) {
+		if (ci > n || rc != GCTF_OK) break;
+		for (int si : splits) {
+			ctx->chunk = ci;
+			ctx->tile_splits = si;
+			if ((rc = ensure_spectrum_buffers(ctx, g, T, ci)) != GCTF_OK) break;
+			if (rp.active) {
+				if ((rc = ensure_rescale_buffers(ctx, rp, ci)) != GCTF_OK) break;
+				if ((rc = ensure_buf(ctx, ctx->padc, sizeof(float) * (size_t)rp.pitch * rp.nny * ci)) != GCTF_OK)
+					break;
+			}
+			float tmin = 1e30f;
+			for (int rep = 0; rep < 3 && rc == GCTF_OK; ++rep) {
+				(void)hipEventRecord(a, ctx->stream);
+				rc = record_spectra(ctx, rp, *tt, dev_imgs, dtype, n, stride);
+				(void)hipEventRecord(b, ctx->stream);
+				if (hipEventSynchronize(b) != hipSuccess) rc = fail(ctx, GCTF_ERR_HIP, "autotune timing");
+				float ms = 0.f;
+				(void)hipEventElapsedTime(&ms, a, b);
+				if (rep > 0) tmin = std::min(tmin, ms);   // the first pass warms caches and plans
+			}
+			if (rc != GCTF_OK) break;
+			if (tmin < best) { best = tmin; bc = ci; bs = si; }
+		}
+	}
+	(void)hipEventDestroy(a);
+	(void)hipEventDestroy(b);
+	ctx->chunk = save_c;
+	ctx->tile_splits = save_s;
+	if (rc != GCTF_OK) return rc;
+	ctx->tuned[shape_key(nx, ny, dtype, pix)] = std::make_pair(bc, bs);
+	if (best_chunk) *best_chunk = bc;
+	if (best_splits) *best_splits = bs;
+	if (best_ms) *best_ms = best;
 	return GCTF_OK;
 }

@@ -94,4 +94,13 @@ struct RingTables {
 	const float* w;        // exp(-16 s^2)
 };
 
+// Per-micrograph search plan (device array indexed by image): the CTF
+// constants at the micrograph's working pixel size, its correlation ring and
+// its 1D bin range.  Micrographs of one batch may differ in all three.
+struct ImgPlan {
+	CtfConsts cc;
+	RingTables ring;
+	float lo1d, hi1d;
+};
+
 }  // namespace gctf

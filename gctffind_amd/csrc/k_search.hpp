@@ -124,14 +124,18 @@ __device__ void search1d_brute(const Search1DParams& p, const float* rad, const 
 	__syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_search1d(Search1DParams p, const float* __restrict__ radial,
-	const double* __restrict__ w1d, SearchState* __restrict__ st)
+__global__ __launch_bounds__(256) void k_search1d(Search1DParams p0, const ImgPlan* __restrict__ plans,
+	const float* __restrict__ radial, const double* __restrict__ w1d, SearchState* __restrict__ st)
 {
 	__shared__ float s_cc[256];
 	__shared__ int s_i[256];
 	__shared__ float res[3];
 	SearchState* S = st + blockIdx.x;
 	if (S->skip) return;
+	Search1DParams p = p0;
+	p.c = plans[blockIdx.x].cc;
+	p.lo = plans[blockIdx.x].lo1d;
+	p.hi = plans[blockIdx.x].hi1d;
 	const float* rad = radial + (long long)blockIdx.x * p.cmpx;
 	const float pix = p.c.pix;
 	float df_min, ext_phase;
@@ -220,10 +224,12 @@ __device__ __forceinline__ CandCtf cand_ctf(const CtfConsts& c, float dfm, float
 // Candidate evaluation.  Grid (ceil(max candidates / CPB), kPixChunks,
 // images); 256 threads.  part[image][cand][chunk] = {sum c*S, sum c*c}.
 template <int CPB>
-__global__ __launch_bounds__(256) void k_cc2d_eval(CtfConsts cc, RingTables ring,
-	const float* __restrict__ ring_spec, const SearchState* __restrict__ st,
+__global__ __launch_bounds__(256) void k_cc2d_eval(const ImgPlan* __restrict__ plans,
+	const float* __restrict__ ring_spec, int kstride, const SearchState* __restrict__ st,
 	double2* __restrict__ part)
 {
+	const CtfConsts cc = plans[blockIdx.z].cc;
+	const RingTables ring = plans[blockIdx.z].ring;
 	__shared__ CandCtf cand[CPB];
 	__shared__ int cvalid[CPB];
 	__shared__ double red[2][CPB][4];
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(256) void k_cc2d_eval(CtfConsts cc, RingTables ring
 	const int chunk = blockIdx.y;
 	const int per = (ring.n + kPixChunks - 1) / kPixChunks;
 	const int k0 = chunk * per, k1 = min(ring.n, k0 + per);
-	const float* sp = ring_spec + (long long)blockIdx.z * ring.n;
+	const float* sp = ring_spec + (long long)blockIdx.z * kstride;
 	float acc_cs[CPB], acc_cc[CPB];
 #pragma unroll
 	for (int j = 0; j < CPB; ++j) { acc_cs[j] = 0.f; acc_cc[j] = 0.f; }
@@ -285,13 +291,11 @@ __global__ __launch_bounds__(256) void k_cc2d_eval(CtfConsts cc, RingTables ring
 	}
 }
 
-// Gather the ring pixels of each spectrum and their FP64 energy.
-__global__ __launch_bounds__(1024) void k_ring_gather(const float* __restrict__ spect, int npix,
-	RingTables ring, float* __restrict__ ring_spec, double* __restrict__ ring_ss)
+// Gather the ring pixels of each spectrum and their FP64 energy: ring
+// spectrum of image i at ring_spec + i * kstride.
+__device__ __forceinline__ void ring_gather(const float* __restrict__ in, const RingTables& ring,
+	float* __restrict__ out, double* __restrict__ ss, double* red)
 {
-	__shared__ double red[1024];
-	const float* in = spect + (long long)blockIdx.x * npix;
-	float* out = ring_spec + (long long)blockIdx.x * ring.n;
 	double acc = 0.0;
 	for (int k = threadIdx.x; k < ring.n; k += 1024) {
 		float v = in[ring.idx[k]];
@@ -304,7 +308,25 @@ __global__ __launch_bounds__(1024) void k_ring_gather(const float* __restrict__ 
 		if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
 		__syncthreads();
 	}
-	if (threadIdx.x == 0) ring_ss[blockIdx.x] = red[0];
+	if (threadIdx.x == 0) *ss = red[0];
+}
+
+__global__ __launch_bounds__(1024) void k_ring_gather(const float* __restrict__ spect, int npix,
+	const ImgPlan* __restrict__ plans, int kstride, float* __restrict__ ring_spec, double* __restrict__ ring_ss)
+{
+	__shared__ double red[1024];
+	const RingTables ring = plans[blockIdx.x].ring;
+	ring_gather(spect + (long long)blockIdx.x * npix, ring, ring_spec + (long long)blockIdx.x * kstride,
+		ring_ss + blockIdx.x, red);
+}
+
+// The same with one ring for the whole batch (the grid search).
+__global__ __launch_bounds__(1024) void k_ring_gather_uniform(const float* __restrict__ spect, int npix,
+	RingTables ring, float* __restrict__ ring_spec, double* __restrict__ ring_ss)
+{
+	__shared__ double red[1024];
+	ring_gather(spect + (long long)blockIdx.x * npix, ring, ring_spec + (long long)blockIdx.x * ring.n,
+		ring_ss + blockIdx.x, red);
 }
 
 // ---- controller -----------------------------------------------------------
@@ -601,9 +623,10 @@ __global__ __launch_bounds__(256) void k_s2d_step(SearchState* __restrict__ st,
 // CTF is evaluated in FP32 with the reference formula (GCalcCTF2D.cu:23-37).
 // Grid (cmpx, images).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_spectral_cc(CtfConsts cc, int cmpx, int cmpy, int width,
-	const float* __restrict__ spect, const SearchState* __restrict__ st, float* __restrict__ shell_cc)
+__global__ __launch_bounds__(256) void k_spectral_cc(const ImgPlan* __restrict__ plans, int cmpx, int cmpy,
+	int width, const float* __restrict__ spect, const SearchState* __restrict__ st, float* __restrict__ shell_cc)
 {
+	const CtfConsts cc = plans[blockIdx.y].cc;
 	__shared__ double red[6][256];
 	const SearchState& S = st[blockIdx.y];
 	if (S.skip) return;
@@ -620,20 +643,32 @@ __global__ __launch_bounds__(256) void k_spectral_cc(CtfConsts cc, int cmpx, int
 	const float* sp = spect + (long long)blockIdx.y * cmpx * cmpy;
 	double a[6] = {0, 0, 0, 0, 0, 0};
 	const int ny = cmpy, nx = min(hi, cmpx);
-	for (int i = threadIdx.x; i < ny * nx; i += 256) {
-		int y = i / nx, x = i % nx;
-		float fy = fabsf((float)y - (float)cmpy * 0.5f);
+	// only the rows and x ranges that can meet lo <= |(x, fy)| < hi are
+	// visited (one pixel of margin); the membership test itself is unchanged
+	const float cyf = (float)cmpy * 0.5f;
+	const int ylo = max(0, (int)floorf(cyf - (float)hi) - 1);
+	const int yhi = min(ny - 1, (int)ceilf(cyf + (float)hi) + 1);
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	for (int y = ylo + wave; y <= yhi; y += 4) {
+		float fy = fabsf((float)y - cyf);
 		if (fy >= (float)hi) continue;
-		float fr = sqrtf((float)(x * x) + fy * fy);
-		if (fr < (float)lo || fr >= (float)hi) continue;
-		float fX = (float)x * 0.5f / (float)(cmpx - 1);
-		float fY = (float)(y - cmpy / 2) / (float)cmpy;
-		float s2 = fX * fX + fY * fY;
-		float th = atanf(fY / (fX + (float)1e-30));
-		float df = mean + sig * cosf(2.0f * (th - az));
-		float c = -sinf(add + kPi2D * cc.wl * s2 * (df - 0.5f * s2 * w2 * cc.cs));
-		float s = sp[y * cmpx + x];
-		a[0] += c; a[1] += s; a[2] += (double)c * s; a[3] += (double)c * c; a[4] += (double)s * s; a[5] += 1.0;
+		const float h2 = (float)hi * (float)hi - fy * fy;
+		const float l2 = (float)lo * (float)lo - fy * fy;
+		const int xmax = min(nx - 1, (int)sqrtf(fmaxf(h2, 0.f)) + 1);
+		const int xmin = l2 > 0.f ? max(0, (int)sqrtf(l2) - 1) : 0;
+		for (int x = xmin + lane; x <= xmax; x += 64) {
+			float fr = sqrtf((float)(x * x) + fy * fy);
+			if (fr < (float)lo || fr >= (float)hi) continue;
+			float fX = (float)x * 0.5f / (float)(cmpx - 1);
+			float fY = (float)(y - cmpy / 2) / (float)cmpy;
+			float s2 = fX * fX + fY * fY;
+			float th = atanf(fY / (fX + (float)1e-30));
+			float df = mean + sig * cosf(2.0f * (th - az));
+			float c = -sinf(add + kPi2D * cc.wl * s2 * (df - 0.5f * s2 * w2 * cc.cs));
+			float sv = sp[y * cmpx + x];
+			a[0] += c; a[1] += sv; a[2] += (double)c * sv; a[3] += (double)c * c; a[4] += (double)sv * sv;
+			a[5] += 1.0;
+		}
 	}
 	for (int j = 0; j < 6; ++j) red[j][threadIdx.x] = a[j];
 	__syncthreads();
@@ -655,11 +690,12 @@ __global__ __launch_bounds__(256) void k_spectral_cc(CtfConsts cc, int cmpx, int
 }
 
 // GSpectralCC2D::DoIt shell pick (:157-171) and m_fCtfRes (CFindDefocus2D.cpp:399).
-__global__ void k_ctf_res(const float* __restrict__ shell_cc, int cmpx, int cmpy, float pix,
-	SearchState* __restrict__ st, int nimg)
+__global__ void k_ctf_res(const float* __restrict__ shell_cc, int cmpx, int cmpy,
+	const ImgPlan* __restrict__ plans, SearchState* __restrict__ st, int nimg)
 {
 	const int b = blockIdx.x * blockDim.x + threadIdx.x;
 	if (b >= nimg || st[b].skip) return;
+	const float pix = plans[b].cc.pix;
 	const float* cc = shell_cc + (long long)b * cmpx;
 	int imax = 1;
 	float fmax = (float)-1e30;

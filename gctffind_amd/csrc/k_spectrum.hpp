@@ -489,17 +489,19 @@ __global__ __launch_bounds__(T / 8) void k_lp_y(const float* __restrict__ q,
 // ---------------------------------------------------------------------------
 // Radial average (GRadialAvg.cu:11-33) from a host-built FP32 sample table.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_radial(const float* __restrict__ spect, int cmpx, int cmpy,
+// Grid (ceil(cmpx / 64), images), 64 threads: thread = bin, samples summed
+// in table order.
+__global__ __launch_bounds__(64) void k_radial(const float* __restrict__ spect, int cmpx, int cmpy,
 	const int* __restrict__ roff, const int* __restrict__ rcnt, const int* __restrict__ ridx,
 	float* __restrict__ radial)
 {
-	const float* in = spect + (long long)blockIdx.x * cmpx * cmpy;
-	for (int r = threadIdx.x; r < cmpx; r += 256) {
-		double s = 0.0;
-		const int o = roff[r], n = rcnt[r];
-		for (int i = 0; i < n; ++i) s += (double)in[ridx[o + i]];
-		radial[(long long)blockIdx.x * cmpx + r] = (float)(s / n);
-	}
+	const float* in = spect + (long long)blockIdx.y * cmpx * cmpy;
+	const int r = blockIdx.x * 64 + threadIdx.x;
+	if (r >= cmpx) return;
+	double s = 0.0;
+	const int o = roff[r], n = rcnt[r];
+	for (int i = 0; i < n; ++i) s += (double)in[ridx[o + i]];
+	radial[(long long)blockIdx.y * cmpx + r] = (float)(s / n);
 }
 
 }  // namespace gctf

@@ -27,6 +27,7 @@ EXPORTS = [
     "gctf_refine", "gctf_spect_size", "gctf_get_half_spect", "gctf_set_half_spect",
     "gctf_full_spectrum", "gctf_estimate_batch", "gctf_tune", "gctf_profile",
     "gctf_profile_report", "gctf_grid_search_batch",
+    "gctf_estimate_list", "gctf_estimate_series", "gctf_autotune",
     "gctf_learned_program_default", "gctf_radial_batch", "gctf_set_proposal_params",
     "gctf_estimate_batch_learned",
     "gctf_net_param_count", "gctf_net_create", "gctf_net_destroy", "gctf_net_error_string",
@@ -63,6 +64,11 @@ class GctfGrid(ctypes.Structure):
 # ring 1/30 - 1/4 1/A)
 GRID_256x32x36 = dict(df_lo=3000.0, df_hi=43000.0, n_df=256, ast_lo=0.0, ast_hi=0.06, n_ast=32,
                       az_lo=-90.0, az_hi=90.0, n_az=36, res_lo=30.0, res_hi=4.0)
+
+
+class GctfImage(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("dtype", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("pix", ctypes.c_float)]
 
 
 class GctfSearchProgram(ctypes.Structure):
@@ -127,6 +133,9 @@ def load():
         "gctf_profile_report": (ci, [vp, ci, fp, fp, fp, fp]),
         "gctf_grid_search_batch": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf,
                                         ctypes.POINTER(GctfGrid), fp, fp, fp, fp]),
+        "gctf_estimate_list": (ci, [vp, ctypes.POINTER(GctfImage), ci, fp]),
+        "gctf_estimate_series": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, fp, ci, fp]),
+        "gctf_autotune": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, fp, fp, fp]),
         "gctf_learned_program_default": (ci, [ctypes.POINTER(GctfSearchProgram)]),
         "gctf_radial_batch": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, vp, ci]),
         "gctf_set_proposal_params": (ci, [vp, fp, ctypes.c_longlong]),
@@ -282,6 +291,32 @@ class Context:
                                                  p(cc), p(top), p(rs)))
         out = [r.as_dict() for r in res]
         return (out, cc, top, rs) if debug else out
+
+    def estimate_list(self, images):
+        """images: sequence of dicts (data: device address, dtype, nx, ny, pix)."""
+        n = len(images)
+        arr = (GctfImage * n)(*[GctfImage(ctypes.c_void_p(im["data"]), im.get("dtype", DTYPE_F32), im["nx"],
+                                          im["ny"], im["pix"]) for im in images])
+        res = (GctfResult * n)()
+        self.check(self.L.gctf_estimate_list(self.h, arr, n, ctypes.cast(res, ctypes.c_void_p)))
+        return [r.as_dict() for r in res]
+
+    def estimate_series(self, dev_ptr, n, nx, ny, pix, tilts, tomo=True, dtype=DTYPE_F32, stride=None):
+        stride = nx * ny if stride is None else stride
+        t = np.ascontiguousarray(tilts, dtype=np.float32)
+        res = (GctfResult * n)()
+        self.check(self.L.gctf_estimate_series(self.h, ctypes.c_void_p(dev_ptr), dtype, n, nx, ny, stride, pix,
+                                               _np_ptr(t), int(bool(tomo)), ctypes.cast(res, ctypes.c_void_p)))
+        return [r.as_dict() for r in res]
+
+    def autotune(self, dev_ptr, n, nx, ny, pix, dtype=DTYPE_F32, stride=None):
+        stride = nx * ny if stride is None else stride
+        c, sp, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        self.check(self.L.gctf_autotune(self.h, ctypes.c_void_p(dev_ptr), dtype, n, nx, ny, stride, pix,
+                                        ctypes.cast(ctypes.pointer(c), ctypes.c_void_p),
+                                        ctypes.cast(ctypes.pointer(sp), ctypes.c_void_p),
+                                        ctypes.cast(ctypes.pointer(ms), ctypes.c_void_p)))
+        return {"chunk": c.value, "tile_splits": sp.value, "ms": ms.value}
 
     def radial_batch(self, dev_ptr, n, nx, ny, pix, dev_out_ptr, ld, dtype=DTYPE_F32, stride=None):
         """Spectrum stages + radial average into a device buffer [n][ld]."""

@@ -158,6 +158,40 @@ GCTF_API int gctf_profile(gctf_ctx* ctx, int enable);
 GCTF_API int gctf_profile_report(gctf_ctx* ctx, int max, char* names, double* total_ms,
 	int* count, int* nout);
 
+/* ---- batching (no reference counterpart) ---------------------------------- */
+
+/* One micrograph of an uneven batch. */
+typedef struct {
+	const void* data;         /* device address, row-major, nx fastest */
+	int dtype;                /* GCTF_DTYPE_F32 or GCTF_DTYPE_F16 */
+	int nx, ny;
+	float pix;                /* A */
+} gctf_image;
+
+/* Micrographs of different shapes, types and pixel sizes in one call (one
+ * HIP graph per list): each is rescaled and tile-averaged on its own plan,
+ * then background removal, low-pass and the plain 1D + 2D search run over
+ * the whole list with per-micrograph CTF constants and rings.  Results
+ * equal gctf_estimate_batch's on each micrograph. */
+GCTF_API int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result* out);
+
+/* A tilt series of n same-shape micrographs in one call (CFindSeriesCtfs,
+ * CFindSeriesCtfs.cpp:133-203): spectra of all images batched, then the
+ * zero-tilt image (CInputFolder::FindZeroTilt, tomo != 0; image 0 otherwise)
+ * gets the full search and every other image, in index order, a Refine
+ * around the best-scoring image so far -- defocus range max(0.5 mean,
+ * 5000 pix^2), astigmatism and azimuth fixed when tomo -- with a full search
+ * when its score is <= 0.  The chain runs on the device, in one graph. */
+GCTF_API int gctf_estimate_series(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny,
+	long long stride, float pix, const float* tilts_deg, int tomo, gctf_result* out);
+
+/* Time the spectrum stages of this batch for every (images per chunk, tile
+ * splits) pair in {1, 2, 4, 8, 16} x {4, 8, 16} and keep the fastest for
+ * this shape (nx, ny, dtype, pix): later batch, series and learned calls of
+ * that shape use it.  Outputs optional. */
+GCTF_API int gctf_autotune(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny,
+	long long stride, float pix, int* best_chunk, int* best_splits, float* best_ms);
+
 /* ---- learned-proposal path (no reference counterpart) -------------------
  * A 1D convolutional network reads each micrograph's radial spectrum (the
  * CFindCtf1D input, CFindCtf1D.cpp:113-117) and proposes its mean defocus

@@ -119,6 +119,8 @@ def test_search_1d_2d_matches_oracle_on_same_spectrum():
     assert abs(rg["df_max"] - ro["df_max"]) <= 5.0
     assert abs(rg["azimuth"] - ro["azimuth"]) <= 0.5
     assert abs(rg["score"] - ro["score"]) <= 1e-4
+    # spectral-CC resolution: the same 0.143 shell (GSpectralCC2D.cu:157-171)
+    assert rg["ctf_res"] == pytest.approx(ro["ctf_res"], rel=1e-6)
 
 
 def test_refine_matches_oracle():
@@ -147,6 +149,7 @@ def test_end_to_end_matches_oracle(n, pix, seed):
     assert abs(rg["df_max"] - ro["df_max"]) <= 5.0
     assert abs(rg["azimuth"] - ro["azimuth"]) <= 0.5
     assert abs(rg["score"] - ro["score"]) <= 1e-4
+    assert rg["ctf_res"] == pytest.approx(ro["ctf_res"], rel=1e-6)
 
 
 def test_batch_equals_single():
@@ -162,7 +165,7 @@ def test_batch_equals_single():
         ctx1, _, _ = _gpu_spectra(imgs[i], pix)
         ctx1.set_phase(0.0, 0.0)
         rs = ctx1.do2d()
-        for k in ("df_min", "df_max", "azimuth", "score"):
+        for k in ("df_min", "df_max", "azimuth", "score", "ctf_res"):
             assert rb[i][k] == rs[k], (i, k)
             assert rb2[i][k] == rs[k], (i, k)
 
