@@ -192,38 +192,51 @@ def timed(torch, run, n, steps, warmup):
 
 
 def evaluate(torch, ctx, a, buf):
-    """Learned path against the plain path (bitwise the FP64 oracle's
-    algorithm) on a.eval held-out micrographs, seeds 2000..: mean-defocus
-    and per-axis defocus differences, fallback and confidence rates, and
-    both paths' error against the generating defocus."""
+    """Learned path against the plain path (the reference algorithm, bitwise
+    the FP64 oracle's on the parity cases) on a.eval held-out micrographs,
+    seeds 2000..: mean-defocus agreement overall, inside the reference's 1D
+    search range (3000 p^2 .. 30000 p^2 at the working pixel size p; outside
+    it the reference itself fails) and by defocus band, agreement within the
+    plain search's own final defocus step (0.2 df x 0.525 / 31, at least
+    20 A), which of the two reached the higher CC, fallback and confidence
+    rates, and both paths' error against the generating defocus."""
     import numpy as np
     n = buf.shape[0]
-    dmean, daxis, fb, conf, e_l, e_p = [], [], [], [], [], []
+    nnx, nny, pitch, pixn = ctx.rescale_setup(a.size, a.size, a.pix)
+    lo1, hi1 = 3000.0 * pixn * pixn, 30000.0 * pixn * pixn
+    rows = []
     for i0 in range(0, a.eval, n):
         m = min(n, a.eval - i0)
         _, truth = make_batch(torch, m, a.size, a.pix, seed0=2000 + i0, out=buf, wide=True)
         lr, props = ctx.estimate_batch_learned(buf.data_ptr(), n, a.size, a.size, a.pix, proposals=True)
         pr = ctx.estimate_batch(buf.data_ptr(), n, a.size, a.size, a.pix)
         for j in range(m):
-            ml = 0.5 * (lr[j]["df_min"] + lr[j]["df_max"])
-            mp = 0.5 * (pr[j]["df_min"] + pr[j]["df_max"])
-            dmean.append(abs(ml - mp))
-            daxis.append(max(abs(lr[j]["df_min"] - pr[j]["df_min"]), abs(lr[j]["df_max"] - pr[j]["df_max"])))
-            fb.append(props[j]["fallback"])
-            conf.append(props[j]["confidence"])
-            e_l.append(abs(ml - truth[j][0]))
-            e_p.append(abs(mp - truth[j][0]))
-    dmean, daxis = np.array(dmean), np.array(daxis)
-    return {"n": len(dmean), "seeds": f"2000..{2000 + len(dmean) - 1}",
+            rows.append((truth[j][0], 0.5 * (lr[j]["df_min"] + lr[j]["df_max"]), 0.5 * (pr[j]["df_min"] + pr[j]["df_max"]),
+                         lr[j]["score"], pr[j]["score"], props[j]["fallback"], props[j]["confidence"]))
+    t, ml, mp, sl, sp, fb, conf = (np.array(c, np.float64) for c in zip(*rows))
+    dd = np.abs(ml - mp)
+    inr = (t >= lo1) & (t <= hi1)
+    step = np.maximum(0.2 * mp * 0.525 / 31.0, 20.0)
+    lowdf = inr & (t < 24000.0)
+    highdf = inr & (t >= 24000.0)
+    dis = inr & (dd > 20.0)
+    r = lambda x: round(float(x), 4)
+    return {"n": int(t.size), "seeds": f"2000..{2000 + t.size - 1}",
             "reference": "plain GPU path (the reference algorithm, bitwise equal to the FP64 oracle on the parity cases)",
-            "frac_mean_df_within_20A": round(float(np.mean(dmean <= 20.0)), 4),
-            "frac_df_min_max_within_20A": round(float(np.mean(daxis <= 20.0)), 4),
-            "p99_abs_dmean_A": round(float(np.percentile(dmean, 99)), 1),
-            "max_abs_dmean_A": round(float(dmean.max()), 1),
-            "fallback_frac": round(float(np.mean(fb)), 4),
+            "frac_within_20A_all": r(np.mean(dd <= 20.0)),
+            "reference_1d_range_A": [round(lo1), round(hi1)],
+            "frac_in_reference_range": r(np.mean(inr)),
+            "frac_within_20A_in_range": r(np.mean(dd[inr] <= 20.0)),
+            "frac_within_20A_in_range_df_below_24000A": r(np.mean(dd[lowdf] <= 20.0)),
+            "frac_within_20A_in_range_df_24000A_up": r(np.mean(dd[highdf] <= 20.0)),
+            "frac_within_plain_final_step_in_range": r(np.mean(dd[inr] <= step[inr])),
+            "frac_learned_cc_ge_plain_where_apart": r(np.mean(sl[dis] >= sp[dis] - 1e-6)) if dis.any() else None,
+            "fallback_frac": r(np.mean(fb)),
             "median_confidence": round(float(np.median(conf)), 3),
-            "median_abs_ddf_vs_truth_learned_A": round(float(np.median(e_l)), 1),
-            "median_abs_ddf_vs_truth_plain_A": round(float(np.median(e_p)), 1)}
+            "median_abs_ddf_vs_truth_learned_A": round(float(np.median(np.abs(ml - t))), 1),
+            "median_abs_ddf_vs_truth_plain_A": round(float(np.median(np.abs(mp - t))), 1),
+            "frac_abs_ddf_vs_truth_over_500A_learned": r(np.mean(np.abs(ml - t) > 500)),
+            "frac_abs_ddf_vs_truth_over_500A_plain": r(np.mean(np.abs(mp - t) > 500))}
 
 
 def geometry(ctx, size, pix, tile=512):

@@ -758,6 +758,7 @@ struct Search2DView {
 	double2* part;
 	RefineArgs* refine;
 	float* shell_cc;
+	float* ctfmap;   // final-parameter CTF per half-spectrum pixel (the low-pass scratch)
 	int T, cx, kstride;
 };
 
@@ -775,6 +776,7 @@ Search2DView view_2d(gctf_ctx* c, int T, int i0)
 	v.part = c->part2d.as<double2>() + (size_t)i0 * kMaxCands * kPixChunks;
 	v.refine = c->refine.as<RefineArgs>() + i0;
 	v.shell_cc = c->shell_cc.as<float>() + (size_t)i0 * v.cx;
+	v.ctfmap = c->q.as<float>() + (size_t)i0 * T * v.cx;
 	return v;
 }
 
@@ -820,7 +822,8 @@ int run_2d_program(gctf_ctx* c, const Search2DView& v, int m, int begin, const S
 		eval(pg.fine_n + 1);
 		step(NX_FINISH);
 	}
-	PROF(c, "k_spectral_cc", k_spectral_cc<<<dim3(v.cx, m), 256, 0, s>>>(v.plans, v.cx, v.T, 10, v.spect, v.st,
+	PROF(c, "k_ctf_map", k_ctf_map<<<dim3(v.T, m), 256, 0, s>>>(v.plans, v.cx, v.T, v.st, v.ctfmap));
+	PROF(c, "k_spectral_cc", k_spectral_cc<<<dim3(v.cx, m), 256, 0, s>>>(v.cx, v.T, 10, v.spect, v.ctfmap, v.st,
 		v.shell_cc));
 	PROF(c, "k_ctf_res", k_ctf_res<<<(m + 63) / 64, 64, 0, s>>>(v.shell_cc, v.cx, v.T, v.plans, v.st, m));
 	HIPCHK(c, hipGetLastError());
@@ -1470,8 +1473,10 @@ int record_batch(gctf_ctx* c, const RescalePlan& rp, SearchPlan& sp, TileTables&
 				sizeof(float) * cx, sizeof(float) * std::min(cx, job.radial_ld), n, hipMemcpyDeviceToDevice, s));
 			return GCTF_OK;
 		}
-		HIPCHK(c, net_forward(c->netb, c->net_params.as<float>(), c->radial.as<float>(), cx, n, true,
-			c->net_out.as<float>(), s));
+		hipError_t ne = hipSuccess;
+		PROF(c, "net_forward", ne = net_forward(c->netb, c->net_params.as<float>(), c->radial.as<float>(), cx, n,
+			true, c->net_out.as<float>(), s));
+		HIPCHK(c, ne);
 		PROF(c, "k_proposal", k_proposal<<<(n + 63) / 64, 64, 0, s>>>(c->net_out.as<float>(), n, job.window,
 			c->props.as<float>(), c->refine.as<RefineArgs>(), c->states.as<SearchState>()));
 		CHK(fill_plans(c, 0, n, sp));
@@ -1717,12 +1722,19 @@ int gctf_estimate_batch_learned(gctf_ctx* ctx, const void* dev_imgs, int dtype, 
 		CHK(get_tile_tables(ctx, ctx->prm.tile_size, &tt));
 		const float range = std::max(ctx->prm.ext_phase_range_deg, 0.0f);
 		SearchState* st = ctx->states.as<SearchState>();
-		k_fallback_prepare<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n, ctx->prm.ext_phase_deg, range);
-		CHK(fill_plans(ctx, 0, n, *sp));
-		CHK(launch_1d(ctx, *tt, 0, n, 0, 0.f, 0.f));
-		CHK(launch_2d(ctx, ctx->prm.tile_size, 0, n, NX_BEGIN_DO2D, range));
-		k_clear_skip<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n);
-		HIPCHK(ctx, hipGetLastError());
+		// one graph for every fallback pattern: which micrographs sit out is data
+		char fkey[160];
+		snprintf(fkey, sizeof fkey, "fallback/%d/%d/%a/%a/%a", n, ctx->prm.tile_size, sp->pix, range,
+			ctx->prm.ext_phase_deg);
+		CHK(run_graph(ctx, fkey, [&]() -> int {
+			k_fallback_prepare<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n, ctx->prm.ext_phase_deg, range);
+			CHK(fill_plans(ctx, 0, n, *sp));
+			CHK(launch_1d(ctx, *tt, 0, n, 0, 0.f, 0.f));
+			CHK(launch_2d(ctx, ctx->prm.tile_size, 0, n, NX_BEGIN_DO2D, range));
+			k_clear_skip<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n);
+			HIPCHK(ctx, hipGetLastError());
+			return GCTF_OK;
+		}));
 		CHK(read_results(ctx, n, out));
 	}
 	if (prop_out) {

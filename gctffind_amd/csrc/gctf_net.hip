@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void k_fc2(const float* __restrict__ h, const 
 
 // Loss per spectrum and its gradient (divided by B).  target[b] =
 // (mean defocus A, astigmatism A, azimuth deg, unused).
-//   (o0 - ln(df / 1e4))^2 + 0.25 (o1 - ast / 1e3)^2
+//   100 (o0 - ln(df / 1e4))^2 + 0.25 (o1 - ast / 1e3)^2
 //   + 0.05 (1 - cos 2 (o2 - az)) + 0.5 BCE(sigmoid(o3), c)
 // with c = 1 when |1e4 exp(o0) - df| <= 200 A (the proposal would put the
 // truth well inside a +-300 A window), else 0.
@@ -250,9 +250,9 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ o, const
 	const float sp_neg = sp_pos - x;                                 // -log p
 	const float bce = c * sp_neg + (1.0f - c) * sp_pos;
 	const float p = 1.0f / (1.0f + expf(-x));
-	const float L = e0 * e0 + 0.25f * e1 * e1 + 0.05f * (1.0f - cosf(d2)) + 0.5f * bce;
+	const float L = 100.0f * e0 * e0 + 0.25f * e1 * e1 + 0.05f * (1.0f - cosf(d2)) + 0.5f * bce;
 	const float inv = 1.0f / (float)B;
-	dO[b * kNetOut + 0] = 2.0f * e0 * inv;
+	dO[b * kNetOut + 0] = 200.0f * e0 * inv;
 	dO[b * kNetOut + 1] = 0.5f * e1 * inv;
 	dO[b * kNetOut + 2] = 0.1f * sinf(d2) * inv;
 	dO[b * kNetOut + 3] = 0.5f * (p - c) * inv;

@@ -619,14 +619,40 @@ __global__ __launch_bounds__(256) void k_s2d_step(SearchState* __restrict__ st,
 }
 
 // ---------------------------------------------------------------------------
-// Spectral CC shells (GSpectralCC2D.cu:12-109) at the final parameters; the
-// CTF is evaluated in FP32 with the reference formula (GCalcCTF2D.cu:23-37).
-// Grid (cmpx, images).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_spectral_cc(const ImgPlan* __restrict__ plans, int cmpx, int cmpy,
-	int width, const float* __restrict__ spect, const SearchState* __restrict__ st, float* __restrict__ shell_cc)
+// The CTF at the final parameters over the half spectrum, in FP32 with the
+// reference formula (GCalcCTF2D.cu:23-37, GSpectralCC2D.cu:12-109), once
+// per pixel: every pixel belongs to about ten shells.  Grid (cmpy, images).
+__global__ __launch_bounds__(256) void k_ctf_map(const ImgPlan* __restrict__ plans, int cmpx, int cmpy,
+	const SearchState* __restrict__ st, float* __restrict__ ctfmap)
 {
+	const SearchState& S = st[blockIdx.y];
+	if (S.skip) return;
 	const CtfConsts cc = plans[blockIdx.y].cc;
+	const int y = blockIdx.x;
+	float ph_rad = S.phase * kD2R2D, az = S.ang * kD2R2D;
+	float dmin = S.dfmean * (1.0f - S.ast) / cc.pix;
+	float dmax = S.dfmean * (1.0f + S.ast) / cc.pix;
+	float mean = 0.5f * (dmin + dmax), sig = 0.5f * (dmax - dmin);
+	float add = cc.amp_phase + ph_rad;
+	float w2 = cc.wl * cc.wl;
+	float* out = ctfmap + ((long long)blockIdx.y * cmpy + y) * cmpx;
+	const float fY = (float)(y - cmpy / 2) / (float)cmpy;
+	for (int x = threadIdx.x; x < cmpx; x += 256) {
+		float fX = (float)x * 0.5f / (float)(cmpx - 1);
+		float s2 = fX * fX + fY * fY;
+		float th = atanf(fY / (fX + (float)1e-30));
+		float df = mean + sig * cosf(2.0f * (th - az));
+		out[x] = -sinf(add + kPi2D * cc.wl * s2 * (df - 0.5f * s2 * w2 * cc.cs));
+	}
+}
+
+// Spectral CC shells (GSpectralCC2D.cu:12-109): Pearson CC of CTF and
+// spectrum over the pixels with lo <= |(x, y - cmpy/2)| < hi of each shell
+// (width 10, clamped at the ends).  Grid (cmpx, images).
+__global__ __launch_bounds__(256) void k_spectral_cc(int cmpx, int cmpy, int width,
+	const float* __restrict__ spect, const float* __restrict__ ctfmap, const SearchState* __restrict__ st,
+	float* __restrict__ shell_cc)
+{
 	__shared__ double red[6][256];
 	const SearchState& S = st[blockIdx.y];
 	if (S.skip) return;
@@ -634,13 +660,8 @@ __global__ __launch_bounds__(256) void k_spectral_cc(const ImgPlan* __restrict__
 	int lo = k - width / 2, hi = lo + width;
 	if (lo < 0) { lo = 0; hi = width; }
 	else if (hi > cmpx) { lo = cmpx - width; hi = cmpx; }
-	float ph_rad = S.phase * kD2R2D, az = S.ang * kD2R2D;
-	float dmin = S.dfmean * (1.0f - S.ast) / cc.pix;
-	float dmax = S.dfmean * (1.0f + S.ast) / cc.pix;
-	float mean = 0.5f * (dmin + dmax), sig = 0.5f * (dmax - dmin);
-	float add = cc.amp_phase + ph_rad;
-	float w2 = cc.wl * cc.wl;
 	const float* sp = spect + (long long)blockIdx.y * cmpx * cmpy;
+	const float* cm = ctfmap + (long long)blockIdx.y * cmpx * cmpy;
 	double a[6] = {0, 0, 0, 0, 0, 0};
 	const int ny = cmpy, nx = min(hi, cmpx);
 	// only the rows and x ranges that can meet lo <= |(x, fy)| < hi are
@@ -659,13 +680,8 @@ __global__ __launch_bounds__(256) void k_spectral_cc(const ImgPlan* __restrict__
 		for (int x = xmin + lane; x <= xmax; x += 64) {
 			float fr = sqrtf((float)(x * x) + fy * fy);
 			if (fr < (float)lo || fr >= (float)hi) continue;
-			float fX = (float)x * 0.5f / (float)(cmpx - 1);
-			float fY = (float)(y - cmpy / 2) / (float)cmpy;
-			float s2 = fX * fX + fY * fY;
-			float th = atanf(fY / (fX + (float)1e-30));
-			float df = mean + sig * cosf(2.0f * (th - az));
-			float c = -sinf(add + kPi2D * cc.wl * s2 * (df - 0.5f * s2 * w2 * cc.cs));
-			float sv = sp[y * cmpx + x];
+			const float c = cm[y * cmpx + x];
+			const float sv = sp[y * cmpx + x];
 			a[0] += c; a[1] += sv; a[2] += (double)c * sv; a[3] += (double)c * c; a[4] += (double)sv * sv;
 			a[5] += 1.0;
 		}

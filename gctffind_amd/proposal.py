@@ -171,7 +171,7 @@ def generate(torch, ctx, n_total, size=3328, pix=1.2, batch=16, seed0=10_000_000
     return radial, target
 
 
-def train(torch, radial, target, epochs=4, batch=256, lr=1e-3, wd=1e-4, seed=0, params=None, log=None,
+def train(torch, radial, target, epochs=30, batch=256, lr=1e-3, wd=1e-4, seed=0, params=None, log=None,
           val_frac=0.01):
     """AdamW with linear warm-up and cosine decay; returns (params, report)."""
     n = radial.shape[0]
@@ -249,7 +249,7 @@ def main(argv=None):
     ap.add_argument("--size", type=int, default=3328)
     ap.add_argument("--pix", type=float, default=1.2)
     ap.add_argument("--gen-batch", type=int, default=16)
-    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)

@@ -14,7 +14,7 @@
  *   fc1     1024 -> 128 (+bias, ReLU), input index c * 16 + l
  *   fc2     128 -> 4
  *   loss    mean over the batch of
- *           (o0 - ln(df / 1e4))^2 + 0.25 (o1 - ast / 1e3)^2
+ *           100 (o0 - ln(df / 1e4))^2 + 0.25 (o1 - ast / 1e3)^2
  *           + 0.05 (1 - cos 2 (o2 - az)) + 0.5 BCE(sigmoid(o3), c),
  *           c = [|1e4 exp(o0) - df| <= 200] (held constant)
  *
@@ -131,11 +131,11 @@ static double loss1(const double* o, const float* tg, double* d)
 	const double sp_neg = sp_pos - x;
 	const double bce = c * sp_neg + (1.0 - c) * sp_pos;
 	const double p = 1.0 / (1.0 + exp(-x));
-	d[0] = 2.0 * e0;
+	d[0] = 200.0 * e0;
 	d[1] = 0.5 * e1;
 	d[2] = 0.1 * sin(d2);
 	d[3] = 0.5 * (p - c);
-	return e0 * e0 + 0.25 * e1 * e1 + 0.05 * (1.0 - cos(d2)) + 0.5 * bce;
+	return 100.0 * e0 * e0 + 0.25 * e1 * e1 + 0.05 * (1.0 - cos(d2)) + 0.5 * bce;
 }
 
 /* backward of conv (cin -> CH, length L): dA_out [CH][L/2] -> grads and dIn [cin][L] */

@@ -35,7 +35,7 @@ def test_loss_closed_form():
     for t in tg.astype(np.float64):
         c = 1.0 if abs(1e4 * math.exp(o[0]) - t[0]) <= 200 else 0.0
         pr = 1 / (1 + math.exp(-o[3]))
-        want += ((o[0] - math.log(t[0] / 1e4)) ** 2 + 0.25 * (o[1] - t[1] / 1e3) ** 2
+        want += (100 * (o[0] - math.log(t[0] / 1e4)) ** 2 + 0.25 * (o[1] - t[1] / 1e3) ** 2
                  + 0.05 * (1 - math.cos(2 * (o[2] - math.radians(t[2]))))
                  - 0.5 * (c * math.log(pr) + (1 - c) * math.log(1 - pr)))
     assert L == pytest.approx(want / 4, rel=1e-12)
