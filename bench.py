@@ -30,6 +30,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TFLOPS = 78.6   # FP32 FMA without packed math: 256 CU x 64 lanes x 2 x 2.4 GHz
+
+# HBM bytes per 4096x4096 micrograph at 1.0 A, measured with two rocprofv3
+# --pmc passes (FETCH_SIZE, WRITE_SIZE) over `bench.py --mode plain` and
+# scaled by the same counters on a 256 MiB device copy (fetch x2.000,
+# write x1.000); the table is profiles/pmc_traffic_plain_batch32_4096sq.md.
+MEASURED_TRAFFIC_PER_IMAGE = {
+    "k_rescale_rows_r2c": 123.15e6, "k_rescale_cols": 103.25e6, "k_rescale_rows_c2r": 94.05e6,
+    "k_block_sums": 49.55e6, "k_tile_rows": 197.05e6, "k_tile_cols": 163.35e6,
+}
+
+# Arithmetic of the FFT kernels per micrograph (5 N log2 N per complex
+# transform of length N, 6 per complex multiply): k_rescale_cols runs 1707
+# columns of FFT 4096 + Bluestein 3412 (two FFTs of 8192, three pointwise
+# products of 8192).
+FFT_FLOP_PER_IMAGE = {
+    "k_rescale_cols": 1707 * (5 * 4096 * 12 + 2 * 5 * 8192 * 13 + 3 * 6 * 8192),
+}
 
 
 def parse():
@@ -48,8 +66,12 @@ def parse():
     ap.add_argument("--eval", type=int, default=1024,
                     help="learned mode: held-out micrographs for the defocus agreement (0: skip)")
     ap.add_argument("--weights", default=None, help="proposal-network weights (.npz)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip gctf_autotune (images per chunk x tile splits chosen by timing)")
     ap.add_argument("--chunk", type=int, default=0, help="images per spectrum chunk (0: library default)")
     ap.add_argument("--splits", type=int, default=0, help="tile-column splits (0: library default)")
+    ap.add_argument("--no-plain-leg", action="store_true",
+                    help="learned mode: do not time the plain path beside it (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB, --steps times (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counters)")
@@ -118,6 +140,9 @@ def main():
         ctx.set_proposal_params(params)
     imgs, truth = make_batch(torch, n, a.size, a.pix, seed0=2000 if learned else 1000, wide=learned)
     ptr = imgs.data_ptr()
+    tuned = None
+    if not a.no_autotune and not a.chunk and not a.splits:
+        tuned = ctx.autotune(ptr, n, a.size, a.size, a.pix)
 
     grid = dict(gctf.GRID_256x32x36)
     plain_run = lambda: ctx.estimate_batch(ptr, n, a.size, a.size, a.pix)
@@ -151,6 +176,7 @@ def main():
              f"micrographs at {a.pix} A"),
             "batch": n, "size": a.size, "pixel_A": a.pix, "tile": 512,
             "path": a.mode,
+            "autotune": tuned,
         },
     }
     if learned:
@@ -158,20 +184,24 @@ def main():
             f"learned-proposal CTF estimation (rescale + 144-tile spectrum + radial average + proposal "
             f"network + +-300 A narrowed 2D search, plain search below confidence 0.8), batch of {n} "
             f"synthetic {a.size}x{a.size} f32 micrographs at {a.pix} A")
-        pv, pms, pres = timed(torch, plain_run, n, a.steps, a.warmup)
-        out["config"]["plain_value"] = round(pv, 2)
-        out["config"]["plain_ms_per_step"] = round(pms, 3)
-        out["config"]["learned_over_plain"] = round(value / pv, 3)
+        pres = None
+        if not a.no_plain_leg:
+            pv, pms, pres = timed(torch, plain_run, n, a.steps, a.warmup)
+            out["config"]["plain_value"] = round(pv, 2)
+            out["config"]["plain_ms_per_step"] = round(pms, 3)
+            out["config"]["learned_over_plain"] = round(value / pv, 3)
         out["config"]["proposal_weights"] = {k: meta.get(k) for k in ("spectra", "micrograph", "epochs", "steps")}
     dfe = [abs(0.5 * (r["df_min"] + r["df_max"]) - t[0]) for r, t in zip(res, truth)]
     out["config"]["median_abs_ddf_vs_truth_A"] = round(float(sorted(dfe)[len(dfe) // 2]), 1)
 
     if not a.no_roofline:
-        out["roofline"], out["kernels"] = roofline(ctx, ptr, n, a, run)
+        out["roofline"], out["kernels"] = roofline(ctx, ptr, n, a, run, tuned)
     img0 = imgs[0].cpu().numpy()
     if learned and a.eval:
         out["config"]["eval"] = evaluate(torch, ctx, a, imgs)
     if not a.no_cpu_baseline:
+        if learned and pres is None:
+            pres = ctx.estimate_batch(ptr, n, a.size, a.size, a.pix)
         out["cpu_baseline"], par = cpu_baseline(img0, a.pix, pres[0] if learned else res[0])
         out["config"]["parity_vs_oracle_img0"] = par
     print(json.dumps(out), flush=True)
@@ -274,7 +304,7 @@ def kernel_bytes(name, g, per_launch_images, nsplit=8):
     return b * c
 
 
-def roofline(ctx, ptr, n, a, run):
+def roofline(ctx, ptr, n, a, run, tuned=None):
     """One extra pass with every kernel bracketed by HIP events on the
     library's stream.  plain: the dominant HBM-bound kernel's algorithmic
     bytes per launch over its average launch duration.  grid: the GEMM's
@@ -284,7 +314,8 @@ def roofline(ctx, ptr, n, a, run):
     run()
     rep = ctx.profile_report()
     ctx.profile(False)
-    chunk = min(a.chunk or 4, n)
+    chunk = min(tuned["chunk"] if tuned else (a.chunk or 4), n)
+    nsplit = tuned["tile_splits"] if tuned else (a.splits or 8)
     batch_kernels = {"k_tile_reduce"}
     rows = {}
     for name, (tot, cnt) in rep.items():
@@ -293,7 +324,7 @@ def roofline(ctx, ptr, n, a, run):
         if name in batch_kernels:
             per = chunk
         avg_ms = tot / cnt
-        nb = kernel_bytes(name, g, per)
+        nb = kernel_bytes(name, g, per, nsplit)
         rows[name] = dict(ms_total=round(tot, 3), launches=cnt, avg_us=round(avg_ms * 1e3, 2),
                           gbs=None if nb is None else round(nb / (avg_ms * 1e-3) / 1e9, 1))
     if a.mode == "grid" and "k_grid_gemm" in rows:
@@ -308,11 +339,21 @@ def roofline(ctx, ptr, n, a, run):
     hbm = {k: v for k, v in rows.items() if v["gbs"] is not None}
     dom = max(hbm, key=lambda k: hbm[k]["ms_total"])
     per = chunk
-    nb = kernel_bytes(dom, g, per)
+    nb = kernel_bytes(dom, g, per, nsplit)
     ach = nb / (rows[dom]["avg_us"] * 1e-6) / 1e9
+    base = dom.split("<")[0]
+    std = a.size == 4096 and a.pix == 1.0
+    traffic = MEASURED_TRAFFIC_PER_IMAGE.get(base) if std else None
     roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": None if traffic is None else int(traffic * per), "kernel": dom,
             "algorithmic_bytes_per_launch": int(nb), "avg_launch_us": rows[dom]["avg_us"]}
+    fl = FFT_FLOP_PER_IMAGE.get(base) if std else None
+    if fl:
+        tf = fl * per / (rows[dom]["avg_us"] * 1e-6) / 1e12
+        roof["valu"] = {"achieved": round(tf, 1), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tf / VALU_PEAK_TFLOPS, 4), "flop_per_launch": int(fl * per),
+                        "note": "FFT-bound: 5 N log2 N per complex transform"}
     return roof, rows
 
 

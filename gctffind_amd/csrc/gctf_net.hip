@@ -435,8 +435,9 @@ hipError_t gemm(NetBuffers* nb, bool bf16, GemmArgs g, hipStream_t s)
 	// split long K so that the grid has enough workgroups
 	const int tiles = ((g.N + 63) / 64) * ((g.M + 63) / 64);
 	int splits = 1;
-	if (g.K >= 2048 && tiles < 256 && nb->part) {
-		splits = std::min((g.K + 1023) / 1024, std::max(1, 512 / tiles));
+	if (tiles < 256 && g.K >= 128 && nb->part) {
+		// at least 64 K per split, enough workgroups to cover the chip
+		splits = std::min(g.K / 64, std::max(1, 512 / tiles));
 		while (splits > 1 && (long long)splits * g.M * g.N > nb->part_cap) --splits;
 	}
 	g.ksplit = ((g.K + splits - 1) / splits + 15) / 16 * 16;
@@ -576,6 +577,9 @@ hipError_t net_buffers_create(int max_batch, bool train, NetBuffers** out)
 	A(&nb->f, kNetFlat * B);
 	A(&nb->h, kNetHidden * B);
 	A(&nb->o, kNetOut * B);
+	// split-K partials of the GEMMs (forward and backward)
+	nb->part_cap = std::max<long long>(1LL << 20, (long long)kNetCh * B * L * 2);
+	A(&nb->part, nb->part_cap);
 	if (train) {
 		A(&nb->dO, kNetOut * B);
 		A(&nb->dh, kNetHidden * B);
@@ -585,8 +589,6 @@ hipError_t net_buffers_create(int max_batch, bool train, NetBuffers** out)
 		A(&nb->dZ, kNetCh * B * L);
 		A(&nb->dcol, (long long)kNetCol * B * L / 2);
 		A(&nb->grad, kNetParams);
-		nb->part_cap = 1LL << 22;
-		A(&nb->part, nb->part_cap);
 		float* lp = nullptr;
 		A(&lp, 2 * B + 2);
 		if (e == hipSuccess) {
