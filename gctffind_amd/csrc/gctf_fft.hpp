@@ -116,6 +116,23 @@ __device__ __forceinline__ void twiddle_powers(float2* u, float2 w)
 	for (int r = 1; r < R; ++r) u[r] = cmul(u[r], p[r]);
 }
 
+// Synchronisation of the NT threads that share one transform.  A transform
+// of 64 threads belongs to a single wavefront: its LDS accesses execute in
+// issue order, so a wavefront-scope fence (which keeps the compiler from
+// moving LDS accesses across it and waits for the outstanding ones) replaces
+// the workgroup barrier, and the other transforms of the workgroup run on.
+template <int NT>
+__device__ __forceinline__ void fft_sync()
+{
+	if constexpr (NT == 64) {
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+	} else {
+		__syncthreads();
+	}
+}
+
 // One Stockham pass of radix R over N points; L = product of the radices of
 // the previous passes.  NT threads (tid in [0, NT)) share the buffer.
 template <int N, int R, int L, int NT, bool INV>
@@ -140,7 +157,7 @@ __device__ __forceinline__ void fft_pass(float2* buf, const float2* __restrict__
 			dft_small<R, INV>(u[b]);
 		}
 	}
-	__syncthreads();
+	fft_sync<NT>();
 #pragma unroll
 	for (int b = 0; b < BPT; ++b) {
 		const int j = tid + b * NT;
@@ -151,7 +168,7 @@ __device__ __forceinline__ void fft_pass(float2* buf, const float2* __restrict__
 			for (int r = 0; r < R; ++r) buf[pidx(base + r * L)] = u[b][r];
 		}
 	}
-	__syncthreads();
+	fft_sync<NT>();
 }
 
 template <int N, int L, int NT, bool INV>
@@ -166,9 +183,11 @@ __device__ __forceinline__ void fft_run(float2* buf, const float2* __restrict__ 
 }
 
 // In-place power-of-two FFT of the N logical elements of buf (padded
-// layout, fft_lds_len(N) entries).  Every thread of the workgroup must call
-// it (it contains __syncthreads); the data must be in buf and visible
-// (barrier) before the call; results are visible after.
+// layout, fft_lds_len(N) entries) by NT threads.  With NT == 64 the threads
+// must form one wavefront and only they synchronise (fft_sync); otherwise
+// every thread of the workgroup must call it.  The data must be in buf and
+// visible (fft_sync<NT> or a barrier) before the call; results are visible
+// to the NT threads after.
 template <int N, int NT, bool INV>
 __device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw, int tid)
 {

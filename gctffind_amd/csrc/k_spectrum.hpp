@@ -234,7 +234,7 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8)) void k_tile_rows(const Tin* _
 		float b = (load_px(src, (long long)pitch + x) - mean) * mask[(ya + 1) * T + x];
 		fb[pidx(x)] = make_float2(a, b);
 	}
-	__syncthreads();
+	fft_sync<FT>();
 	fft_lds<T, FT, false>(fb, tw, lane);
 	for (int k = lane; k < CX; k += FT) {
 		float2 z = fb[pidx(k)];
@@ -282,14 +282,14 @@ __global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols(const float2* __res
 			const float2* col = src + ((long long)t * CX + kx) * T;
 			for (int y = lane; y < T; y += FT) fb[pidx(y)] = col[y];
 		}
-		__syncthreads();
+		fft_sync<FT>();
 		fft_lds<T, FT, false>(fb, tw, lane);
 #pragma unroll
 		for (int i = 0; i < PER; ++i) {
 			float2 c = fb[pidx(lane + i * FT)];
 			acc[i] = fmaf(sqrtf(fmaf(c.x, c.x, c.y * c.y)), fac, acc[i]);
 		}
-		__syncthreads();
+		fft_sync<FT>();
 	}
 	if (!live) return;
 	float* dst = part + ((long long)blockIdx.z * nsplit + s) * (long long)T * CX;
@@ -435,14 +435,14 @@ __global__ __launch_bounds__(2 * (T / 8)) void k_lp_x(const float* __restrict__ 
 		}
 		fb[pidx(xd)] = make_float2(v[0], v[1]);
 	}
-	__syncthreads();
+	fft_sync<FT>();
 	fft_lds<T, FT, false>(fb, tw, lane);
 	for (int k = lane; k < T; k += FT) {
 		float f = h1[k];
 		float2 z = fb[pidx(k)];
 		fb[pidx(k)] = make_float2(z.x * f, z.y * f);
 	}
-	__syncthreads();
+	fft_sync<FT>();
 	fft_lds<T, FT, true>(fb, tw, lane);
 	float* out = q + (long long)blockIdx.y * T * CX;
 	const float inv = 1.0f / (float)T;
