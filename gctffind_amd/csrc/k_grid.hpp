@@ -237,18 +237,32 @@ __global__ __launch_bounds__(256) void k_grid_topk(const float* __restrict__ cc,
 	int ix[kGridTopK];
 #pragma unroll
 	for (int i = 0; i < kGridTopK; ++i) { v[i] = -3.0e38f; ix[i] = 0x7fffffff; }
-	for (int mm = r0; mm < r1; ++mm) {
-		const float x = cc[(long long)mm * np + col];
-		if (!(x > v[kGridTopK - 1])) continue;
-		// insert (strictly greater values move up; equal values keep order)
-		float cv = x;
-		int ci = mm;
+	// rows in groups of U: the loads of a group are independent and issue
+	// back to back; the insertions then run in row order
+	constexpr int U = 16;
+	for (int m0 = r0; m0 < r1; m0 += U) {
+		float xs[U];
 #pragma unroll
-		for (int i = 0; i < kGridTopK; ++i) {
-			if (cv > v[i]) {
-				float tv = v[i]; int ti = ix[i];
-				v[i] = cv; ix[i] = ci;
-				cv = tv; ci = ti;
+		for (int u = 0; u < U; ++u) {
+			const int mm = m0 + u;
+			xs[u] = mm < r1 ? cc[(long long)mm * np + col] : -3.0e38f;
+		}
+#pragma unroll
+		for (int u = 0; u < U; ++u) {
+			if (!(xs[u] > v[kGridTopK - 1])) continue;
+			// branch-free insertion (strictly greater values move up; equal
+			// values keep their order)
+			float cv = xs[u];
+			int ci = m0 + u;
+#pragma unroll
+			for (int i = 0; i < kGridTopK; ++i) {
+				const bool gt = cv > v[i];
+				const float tv = v[i];
+				const int ti = ix[i];
+				v[i] = gt ? cv : tv;
+				ix[i] = gt ? ci : ti;
+				cv = gt ? tv : cv;
+				ci = gt ? ti : ci;
 			}
 		}
 	}
