@@ -86,3 +86,11 @@ def test_weight_file_round_trip(tmp_path):
     proposal.save_params(str(f), p, {"spectra": 12, "note": "x"})
     q, meta = proposal.load_params(str(f))
     assert np.array_equal(p, q) and meta["spectra"] == 12
+
+
+def test_shipped_weights():
+    """The trained parameters in the package: the network's size, finite,
+    and the training record (spectra from the library's own pipeline)."""
+    p, meta = proposal.load_params()
+    assert p.size == proposal.n_params() and np.all(np.isfinite(p))
+    assert meta["spectra"] >= 500_000 and meta["val"]["median_abs_ddf_A"] < 50.0
