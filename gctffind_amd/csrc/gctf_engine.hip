@@ -562,7 +562,7 @@ int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long lon
 	float2* A = c->cbufA.as<float2>();
 	float2* B = c->cbufB.as<float2>();
 	const long long stA = std::max(sa, sb), stB = std::max(sa, sb);
-	// 1. rows R2C with x crop
+	// 1. rows R2C with x crop into A: A[y][kx]
 	{
 		const int m = rp.rows->m;
 		dim3 grid((rp.ny + 1) / 2, n);
@@ -572,35 +572,30 @@ int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long lon
 			else if (m <= 8192) k_rescale_rows_r2c<Tin, 1024, 8192><<<grid, 1024, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
 			else k_rescale_rows_r2c<Tin, 1024, 16384><<<grid, 1024, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA));
 	}
-	// 2. transpose (ny x ncx) -> (ncx x ny)
+	// transpose (ny x ncx) -> B[kx][y]
 	{
 		dim3 grid((ncx + 31) / 32, (rp.ny + 31) / 32, n);
 		PROF(c, "k_transpose_c", k_transpose_c<<<grid, 256, 0, s>>>(A, stA, rp.ny, ncx, B, stB));
 	}
-	// 3. columns: forward ny, crop, inverse nny
+	// 2. columns: forward ny, crop, inverse nny, written row-major into A: A[y'][kx]
 	{
 		const int m = std::max(rp.cf->m, rp.ci->m);
-		dim3 grid(ncx, n);
+		dim3 grid(8 * ((ncx + 7) / 8), n);
 		Fft1dPlan pf = rp.cf->plan(), pi = rp.ci->plan();
-		PROF(c, "k_rescale_cols", if (m <= 2048) k_rescale_cols<256, 2048><<<grid, 256, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
-			else if (m <= 4096) k_rescale_cols<512, 4096><<<grid, 512, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
-			else if (m <= 8192) k_rescale_cols<1024, 8192><<<grid, 1024, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA);
-			else k_rescale_cols<1024, 16384><<<grid, 1024, 0, s>>>(B, stB, rp.ny, rp.nny, pf, pi, A, stA));
+		PROF(c, "k_rescale_cols", if (m <= 2048) k_rescale_cols<256, 2048><<<grid, 256, 0, s>>>(B, stB, rp.ny, rp.nny, ncx, pf, pi, A, stA);
+			else if (m <= 4096) k_rescale_cols<512, 4096><<<grid, 512, 0, s>>>(B, stB, rp.ny, rp.nny, ncx, pf, pi, A, stA);
+			else if (m <= 8192) k_rescale_cols<1024, 8192><<<grid, 1024, 0, s>>>(B, stB, rp.ny, rp.nny, ncx, pf, pi, A, stA);
+			else k_rescale_cols<1024, 16384><<<grid, 1024, 0, s>>>(B, stB, rp.ny, rp.nny, ncx, pf, pi, A, stA));
 	}
-	// 4. transpose (ncx x nny) -> (nny x ncx)
-	{
-		dim3 grid((rp.nny + 31) / 32, (ncx + 31) / 32, n);
-		PROF(c, "k_transpose_c", k_transpose_c<<<grid, 256, 0, s>>>(A, stA, ncx, rp.nny, B, stB));
-	}
-	// 5. rows C2R into the padded image
+	// 3. rows C2R into the padded image
 	{
 		const int m = rp.c2r->m;
 		dim3 grid((rp.nny + 1) / 2, n);
 		Fft1dPlan p = rp.c2r->plan();
-		PROF(c, "k_rescale_rows_c2r", if (m <= 2048) k_rescale_rows_c2r<256, 2048><<<grid, 256, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
-			else if (m <= 4096) k_rescale_rows_c2r<512, 4096><<<grid, 512, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
-			else if (m <= 8192) k_rescale_rows_c2r<1024, 8192><<<grid, 1024, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
-			else k_rescale_rows_c2r<1024, 16384><<<grid, 1024, 0, s>>>(B, stB, rp.nny, rp.nnx, p, out, out_stride, rp.pitch));
+		PROF(c, "k_rescale_rows_c2r", if (m <= 2048) k_rescale_rows_c2r<256, 2048><<<grid, 256, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
+			else if (m <= 4096) k_rescale_rows_c2r<512, 4096><<<grid, 512, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
+			else if (m <= 8192) k_rescale_rows_c2r<1024, 8192><<<grid, 1024, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
+			else k_rescale_rows_c2r<1024, 16384><<<grid, 1024, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch));
 	}
 	HIPCHK(c, hipGetLastError());
 	return GCTF_OK;

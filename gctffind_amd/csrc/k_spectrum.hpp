@@ -13,6 +13,18 @@ __device__ __forceinline__ float load_px(const _Float16* p, long long i) { retur
 // Rescale (CRescaleImage.cpp:47-99; CCufft2D.cpp:23-114; GFtResize2D.cu:7-32)
 // ---------------------------------------------------------------------------
 
+// The 8 XCDs of the chip take workgroups round-robin (XCD = linear id mod
+// 8).  xcd_group(b, n) maps a grid of 8 ceil(n / 8) workgroups so that each
+// XCD works through one contiguous range of [0, n): neighbouring rows or
+// columns then meet in the same L2, which merges their partial-line writes.
+// Returns -1 for the padding workgroups.
+__device__ __forceinline__ int xcd_group(int b, int n)
+{
+	const int per = (n + 7) / 8;
+	const int i = (b & 7) * per + (b >> 3);
+	return i < n ? i : -1;
+}
+
 // Real rows -> half-spectrum rows kept for kx < ncx_out.  Two rows share one
 // complex transform (z = a + i b).  Grid (ceil(ny/2), images).
 template <typename Tin, int NT, int MAXN>
@@ -64,17 +76,20 @@ __global__ __launch_bounds__(256) void k_transpose_c(const float2* __restrict__ 
 	}
 }
 
-// Columns (rows of the transposed spectrum): forward length ny, crop to
-// nny rows (ky <= nny/2 from the top, the rest from the bottom), inverse
-// length nny.  Grid (ncx, images).
+// Columns (rows of the transposed spectrum in[kx][y]): forward length ny,
+// crop to nny rows (ky <= nny/2 from the top, the rest from the bottom),
+// inverse length nny, written back row-major: out[y'][kx] (ncx entries per
+// y').  Grid (8 ceil(ncx / 8), images).
 template <int NT, int MAXN>
 __global__ __launch_bounds__(NT) void k_rescale_cols(const float2* __restrict__ in,
-	long long in_stride, int ny, int nny, Fft1dPlan pf, Fft1dPlan pi,
+	long long in_stride, int ny, int nny, int ncx, Fft1dPlan pf, Fft1dPlan pi,
 	float2* __restrict__ out, long long out_stride)
 {
 	__shared__ float2 buf[fft_lds_len(MAXN)];
 	const int tid = threadIdx.x;
-	const float2* src = in + blockIdx.y * in_stride + (long long)blockIdx.x * ny;
+	const int kx = xcd_group(blockIdx.x, ncx);
+	if (kx < 0) return;
+	const float2* src = in + blockIdx.y * in_stride + (long long)kx * ny;
 	for (int y = tid; y < ny; y += NT) buf[pidx(y)] = src[y];
 	__syncthreads();
 	fft_any_lds<NT, MAXN, false>(buf, pf, tid);
@@ -97,8 +112,8 @@ __global__ __launch_bounds__(NT) void k_rescale_cols(const float2* __restrict__ 
 	}
 	__syncthreads();
 	fft_any_lds<NT, MAXN, true>(buf, pi, tid);
-	float2* dst = out + blockIdx.y * out_stride + (long long)blockIdx.x * nny;
-	for (int y = tid; y < nny; y += NT) dst[y] = buf[pidx(y)];
+	float2* dst = out + blockIdx.y * out_stride + kx;
+	for (int y = tid; y < nny; y += NT) dst[(long long)y * ncx] = buf[pidx(y)];
 }
 
 // Half-spectrum rows -> real rows of length nnx (C2R; imaginary parts of the
