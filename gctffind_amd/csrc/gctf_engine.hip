@@ -119,7 +119,9 @@ double round_edge(int x, int y, int tile)
 struct TileTables {
 	int T = 0;
 	DevBuf tw, mask, h1, roff, rcnt, ridx, w1d;
+	DevBuf boff, bcnt, bidx;   // pixels of each integer radius bin (spectral CC)
 	int nsamples = 0;
+	int nbins = 0;
 };
 
 struct RescalePlan {
@@ -177,7 +179,7 @@ struct gctf_ctx {
 	int cap = 0;  // images the per-image buffers hold
 	DevBuf cbufA, cbufB, padc, bsums, rt, tpart;
 	DevBuf raw, hsum, spect, mom, q, radial, ring_spec, ring_ss, states, part2d, refine, shell_cc, fullsp,
-		moments1, plans;
+		moments1, plans, shell_bins;
 	int kstride = 0;  // ring pixels reserved per image in ring_spec
 	DevBuf g_bt, g_fn, g_gt, g_cc, g_part, g_top, g_rs, g_rescored;
 	// learned path: proposal-network parameters, activations, outputs
@@ -330,6 +332,31 @@ void radial_table(int cmpx, int cmpy, std::vector<int>& off, std::vector<int>& c
 	}
 }
 
+// Pixels of the half spectrum by integer radius bin floor(|(x, y - cmpy/2)|),
+// the radius in the float arithmetic of the spectral-CC shells
+// (GSpectralCC2D.cu:26-40).
+void shell_bin_table(int cmpx, int cmpy, std::vector<int>& off, std::vector<int>& cnt, std::vector<int>& idx)
+{
+	std::vector<std::vector<int>> bins;
+	for (int y = 0; y < cmpy; ++y) {
+		const float fy = fabsf((float)y - (float)cmpy * 0.5f);
+		for (int x = 0; x < cmpx; ++x) {
+			const float fr = sqrtf((float)(x * x) + fy * fy);
+			const int r = (int)fr;
+			if (r >= (int)bins.size()) bins.resize(r + 1);
+			bins[r].push_back(y * cmpx + x);
+		}
+	}
+	off.assign(bins.size(), 0);
+	cnt.assign(bins.size(), 0);
+	idx.clear();
+	for (size_t r = 0; r < bins.size(); ++r) {
+		off[r] = (int)idx.size();
+		cnt[r] = (int)bins[r].size();
+		idx.insert(idx.end(), bins[r].begin(), bins[r].end());
+	}
+}
+
 int get_tile_tables(gctf_ctx* c, int T, TileTables** out)
 {
 	auto it = c->tiles.find(T);
@@ -354,6 +381,15 @@ int get_tile_tables(gctf_ctx* c, int T, TileTables** out)
 	}
 	std::vector<int> off, cnt, idx;
 	radial_table(cx, T, off, cnt, idx);
+	std::vector<int> boff, bcnt, bidx;
+	shell_bin_table(cx, T, boff, bcnt, bidx);
+	t->nbins = (int)boff.size();
+	CHK(ensure_buf(c, t->boff, sizeof(int) * boff.size()));
+	CHK(ensure_buf(c, t->bcnt, sizeof(int) * bcnt.size()));
+	CHK(ensure_buf(c, t->bidx, sizeof(int) * bidx.size()));
+	HIPCHK(c, hipMemcpy(t->boff.p, boff.data(), sizeof(int) * boff.size(), hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(t->bcnt.p, bcnt.data(), sizeof(int) * bcnt.size(), hipMemcpyHostToDevice));
+	HIPCHK(c, hipMemcpy(t->bidx.p, bidx.data(), sizeof(int) * bidx.size(), hipMemcpyHostToDevice));
 	std::vector<double> w1d(cx);
 	for (int x = 0; x < cx; ++x) {
 		double f = (double)x / ((double)cx - 1.0);  // GCC1D.cu:36-39, B = 1
@@ -529,6 +565,9 @@ int ensure_image_buffers(gctf_ctx* c, int n, int T, int ring_n)
 	CHK(ensure_buf(c, c->refine, sizeof(RefineArgs) * n));
 	CHK(ensure_buf(c, c->plans, sizeof(ImgPlan) * n));
 	CHK(ensure_buf(c, c->shell_cc, sizeof(float) * cx * n));
+	// integer radius bins of the half spectrum: floor(sqrt(cx^2 + (T/2)^2)) + 1 at most
+	const size_t nb = (size_t)std::sqrt((double)cx * cx + (double)(T / 2) * (T / 2)) + 2;
+	CHK(ensure_buf(c, c->shell_bins, sizeof(double) * 6 * nb * n));
 	size_t old = c->states.bytes;
 	CHK(ensure_buf(c, c->states, sizeof(SearchState) * n));
 	if (c->states.bytes != old) {
@@ -754,6 +793,9 @@ struct Search2DView {
 	RefineArgs* refine;
 	float* shell_cc;
 	float* ctfmap;   // final-parameter CTF per half-spectrum pixel (the low-pass scratch)
+	double* bins;    // spectral-CC sums per integer radius bin
+	const int *boff, *bcnt, *bidx;
+	int nbins;
 	int T, cx, kstride;
 };
 
@@ -772,6 +814,12 @@ Search2DView view_2d(gctf_ctx* c, int T, int i0)
 	v.refine = c->refine.as<RefineArgs>() + i0;
 	v.shell_cc = c->shell_cc.as<float>() + (size_t)i0 * v.cx;
 	v.ctfmap = c->q.as<float>() + (size_t)i0 * T * v.cx;
+	const TileTables& tt = *c->tiles.at(T);
+	v.nbins = tt.nbins;
+	v.boff = tt.boff.as<int>();
+	v.bcnt = tt.bcnt.as<int>();
+	v.bidx = tt.bidx.as<int>();
+	v.bins = c->shell_bins.as<double>() + (size_t)i0 * 6 * tt.nbins;
 	return v;
 }
 
@@ -818,7 +866,9 @@ int run_2d_program(gctf_ctx* c, const Search2DView& v, int m, int begin, const S
 		step(NX_FINISH);
 	}
 	PROF(c, "k_ctf_map", k_ctf_map<<<dim3(v.T, m), 256, 0, s>>>(v.plans, v.cx, v.T, v.st, v.ctfmap));
-	PROF(c, "k_spectral_cc", k_spectral_cc<<<dim3(v.cx, m), 256, 0, s>>>(v.cx, v.T, 10, v.spect, v.ctfmap, v.st,
+	PROF(c, "k_shell_bins", k_shell_bins<<<dim3(v.nbins, m), 64, 0, s>>>(v.spect, v.ctfmap, v.T * v.cx, v.boff,
+		v.bcnt, v.bidx, v.nbins, v.st, v.bins));
+	PROF(c, "k_shell_cc", k_shell_cc<<<dim3((v.cx + 63) / 64, m), 64, 0, s>>>(v.bins, v.nbins, v.cx, 10, v.st,
 		v.shell_cc));
 	PROF(c, "k_ctf_res", k_ctf_res<<<(m + 63) / 64, 64, 0, s>>>(v.shell_cc, v.cx, v.T, v.plans, v.st, m));
 	HIPCHK(c, hipGetLastError());

@@ -648,61 +648,56 @@ __global__ __launch_bounds__(256) void k_ctf_map(const ImgPlan* __restrict__ pla
 
 // Spectral CC shells (GSpectralCC2D.cu:12-109): Pearson CC of CTF and
 // spectrum over the pixels with lo <= |(x, y - cmpy/2)| < hi of each shell
-// (width 10, clamped at the ends).  Grid (cmpx, images).
-__global__ __launch_bounds__(256) void k_spectral_cc(int cmpx, int cmpy, int width,
-	const float* __restrict__ spect, const float* __restrict__ ctfmap, const SearchState* __restrict__ st,
-	float* __restrict__ shell_cc)
+// (width 10, clamped at the ends).  The shell bounds are integers, so a
+// pixel with radius fr belongs to shell [lo, hi) exactly when
+// lo <= floor(fr) < hi: the six sums of each integer radius bin are formed
+// once (k_shell_bins, from a host table listing the pixels of every bin,
+// radii in the kernel's float arithmetic), and each shell adds its bins.
+// Grid (nbins, images), one wavefront per bin; bins[image][bin][6].
+__global__ __launch_bounds__(64) void k_shell_bins(const float* __restrict__ spect,
+	const float* __restrict__ ctfmap, int npix, const int* __restrict__ boff, const int* __restrict__ bcnt,
+	const int* __restrict__ bidx, int nbins, const SearchState* __restrict__ st, double* __restrict__ bins)
 {
-	__shared__ double red[6][256];
-	const SearchState& S = st[blockIdx.y];
-	if (S.skip) return;
-	const int k = blockIdx.x;
+	const int b = blockIdx.x, img = blockIdx.y;
+	if (st[img].skip) return;
+	const float* sp = spect + (long long)img * npix;
+	const float* cm = ctfmap + (long long)img * npix;
+	const int o = boff[b], n = bcnt[b];
+	double a[6] = {0, 0, 0, 0, 0, 0};
+	for (int i = threadIdx.x; i < n; i += 64) {
+		const int p = bidx[o + i];
+		const float c = cm[p], sv = sp[p];
+		a[0] += c; a[1] += sv; a[2] += (double)c * sv; a[3] += (double)c * c; a[4] += (double)sv * sv;
+	}
+	a[5] = (double)n;
+#pragma unroll
+	for (int j = 0; j < 5; ++j)
+		for (int off = 32; off > 0; off >>= 1) a[j] += __shfl_xor(a[j], off);
+	if (threadIdx.x < 6) bins[((long long)img * nbins + b) * 6 + threadIdx.x] = a[threadIdx.x];
+}
+
+// Shell k = the bins [lo, hi); grid (ceil(cmpx / 64), images).
+__global__ __launch_bounds__(64) void k_shell_cc(const double* __restrict__ bins, int nbins, int cmpx, int width,
+	const SearchState* __restrict__ st, float* __restrict__ shell_cc)
+{
+	const int k = blockIdx.x * 64 + threadIdx.x, img = blockIdx.y;
+	if (k >= cmpx || st[img].skip) return;
 	int lo = k - width / 2, hi = lo + width;
 	if (lo < 0) { lo = 0; hi = width; }
 	else if (hi > cmpx) { lo = cmpx - width; hi = cmpx; }
-	const float* sp = spect + (long long)blockIdx.y * cmpx * cmpy;
-	const float* cm = ctfmap + (long long)blockIdx.y * cmpx * cmpy;
 	double a[6] = {0, 0, 0, 0, 0, 0};
-	const int ny = cmpy, nx = min(hi, cmpx);
-	// only the rows and x ranges that can meet lo <= |(x, fy)| < hi are
-	// visited (one pixel of margin); the membership test itself is unchanged
-	const float cyf = (float)cmpy * 0.5f;
-	const int ylo = max(0, (int)floorf(cyf - (float)hi) - 1);
-	const int yhi = min(ny - 1, (int)ceilf(cyf + (float)hi) + 1);
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	for (int y = ylo + wave; y <= yhi; y += 4) {
-		float fy = fabsf((float)y - cyf);
-		if (fy >= (float)hi) continue;
-		const float h2 = (float)hi * (float)hi - fy * fy;
-		const float l2 = (float)lo * (float)lo - fy * fy;
-		const int xmax = min(nx - 1, (int)sqrtf(fmaxf(h2, 0.f)) + 1);
-		const int xmin = l2 > 0.f ? max(0, (int)sqrtf(l2) - 1) : 0;
-		for (int x = xmin + lane; x <= xmax; x += 64) {
-			float fr = sqrtf((float)(x * x) + fy * fy);
-			if (fr < (float)lo || fr >= (float)hi) continue;
-			const float c = cm[y * cmpx + x];
-			const float sv = sp[y * cmpx + x];
-			a[0] += c; a[1] += sv; a[2] += (double)c * sv; a[3] += (double)c * c; a[4] += (double)sv * sv;
-			a[5] += 1.0;
-		}
+	const double* bb = bins + (long long)img * nbins * 6;
+	for (int r = lo; r < hi && r < nbins; ++r)
+#pragma unroll
+		for (int j = 0; j < 6; ++j) a[j] += bb[r * 6 + j];
+	float out = 0.f;
+	const double n = a[5];
+	if (n > 0) {
+		double mc = a[0] / n, ms = a[1] / n, mcs = a[2] / n;
+		double vc = a[3] / n - mc * mc, vs = a[4] / n - ms * ms;
+		if (vc > 0 && vs > 0) out = (float)((mcs - mc * ms) / sqrt(vc * vs));
 	}
-	for (int j = 0; j < 6; ++j) red[j][threadIdx.x] = a[j];
-	__syncthreads();
-	for (int o = 128; o > 0; o >>= 1) {
-		if (threadIdx.x < o)
-			for (int j = 0; j < 6; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + o];
-		__syncthreads();
-	}
-	if (threadIdx.x == 0) {
-		double n = red[5][0];
-		float out = 0.f;
-		if (n > 0) {
-			double mc = red[0][0] / n, ms = red[1][0] / n, mcs = red[2][0] / n;
-			double vc = red[3][0] / n - mc * mc, vs = red[4][0] / n - ms * ms;
-			if (vc > 0 && vs > 0) out = (float)((mcs - mc * ms) / sqrt(vc * vs));
-		}
-		shell_cc[(long long)blockIdx.y * cmpx + k] = out;
-	}
+	shell_cc[(long long)img * cmpx + k] = out;
 }
 
 // GSpectralCC2D::DoIt shell pick (:157-171) and m_fCtfRes (CFindDefocus2D.cpp:399).
