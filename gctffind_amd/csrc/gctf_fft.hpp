@@ -231,10 +231,20 @@ __device__ __forceinline__ void fft_pass_rt(float2* buf, const float2* __restric
 }
 
 // In-place power-of-two FFT of runtime length n <= MAXN; same contract as
-// fft_lds.
+// fft_lds.  The lengths MAXN and MAXN/2 run the compile-time transform (the
+// same arithmetic, with its LDS addresses computed once per thread instead
+// of once per pass); other lengths loop over runtime passes.
 template <int NT, int MAXN, bool INV>
 __device__ __forceinline__ void fft_lds_rt(float2* buf, int n, const float2* __restrict__ tw, int tid)
 {
+	if (n == MAXN) {
+		fft_lds<MAXN, NT, INV>(buf, tw, tid);
+		return;
+	}
+	if (n == MAXN / 2) {
+		fft_lds<MAXN / 2, NT, INV>(buf, tw, tid);
+		return;
+	}
 	int L = 1;
 	while (L < n) {
 		const int rem = n / L;

@@ -224,8 +224,11 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8)) void k_tile_rows(const Tin* _
 	constexpr int NT = NG * FT;
 	constexpr int CX = T / 2 + 1;
 	constexpr int FL = fft_lds_len(T);
+	constexpr int SP = ROWS + 1;           // stage pitch: lanes of a wave write
+	                                       // k-consecutive entries, an odd pitch
+	                                       // keeps them off one LDS bank
 	__shared__ float2 buf[NG * FL];
-	__shared__ float2 stage[CX * ROWS];
+	__shared__ float2 stage[CX * SP];
 	const int rb = blockIdx.x % (T / ROWS);
 	const int t = blockIdx.x / (T / ROWS);
 	const int tx = t % g.ntx, ty = t / g.ntx;
@@ -256,14 +259,14 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8)) void k_tile_rows(const Tin* _
 		float2 zc = fb[pidx((T - k) & (T - 1))];
 		float2 A = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
 		float2 B = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
-		stage[k * ROWS + 2 * grp] = A;
-		stage[k * ROWS + 2 * grp + 1] = B;
+		stage[k * SP + 2 * grp] = A;
+		stage[k * SP + 2 * grp + 1] = B;
 	}
 	__syncthreads();
 	float2* dst = rt + blockIdx.y * rt_stride + (long long)t * CX * T + rb * ROWS;
 	for (int i = threadIdx.x; i < CX * ROWS; i += NT) {
 		int k = i / ROWS, yy = i % ROWS;
-		dst[(long long)k * T + yy] = stage[i];
+		dst[(long long)k * T + yy] = stage[k * SP + yy];
 	}
 }
 
