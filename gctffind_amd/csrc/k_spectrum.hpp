@@ -212,9 +212,10 @@ __device__ __forceinline__ double block_sum(const double* bs, int b)
 // mean, apply the round-edge mask, R2C transform along x (two rows per
 // complex transform) and write the half-spectrum transposed:
 // rt[image][tile][kx][y].  Grid (ntiles * T/ROWS, images), (ROWS/2)*(T/8)
-// threads.
+// threads.  The transposing stage reuses the transforms' LDS (the split
+// halves wait in registers across a barrier), so four workgroups fit a CU.
 template <typename Tin, int T, int ROWS>
-__global__ __launch_bounds__((ROWS / 2) * (T / 8)) void k_tile_rows(const Tin* __restrict__ img,
+__global__ __launch_bounds__((ROWS / 2) * (T / 8), 8) void k_tile_rows(const Tin* __restrict__ img,
 	long long img_stride, int pitch, TileGrid g, const double* __restrict__ sums,
 	const float* __restrict__ mask, const float2* __restrict__ tw,
 	float2* __restrict__ rt, long long rt_stride)
@@ -224,11 +225,13 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8)) void k_tile_rows(const Tin* _
 	constexpr int NT = NG * FT;
 	constexpr int CX = T / 2 + 1;
 	constexpr int FL = fft_lds_len(T);
+	constexpr int KP = (CX + FT - 1) / FT; // half-spectrum entries per thread
 	constexpr int SP = ROWS + 1;           // stage pitch: lanes of a wave write
 	                                       // k-consecutive entries, an odd pitch
 	                                       // keeps them off one LDS bank
+	static_assert(CX * SP <= NG * FL, "stage must fit in the transform buffers");
 	__shared__ float2 buf[NG * FL];
-	__shared__ float2 stage[CX * SP];
+	float2* stage = buf;
 	const int rb = blockIdx.x % (T / ROWS);
 	const int t = blockIdx.x / (T / ROWS);
 	const int tx = t % g.ntx, ty = t / g.ntx;
@@ -254,13 +257,25 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8)) void k_tile_rows(const Tin* _
 	}
 	fft_sync<FT>();
 	fft_lds<T, FT, false>(fb, tw, lane);
-	for (int k = lane; k < CX; k += FT) {
-		float2 z = fb[pidx(k)];
-		float2 zc = fb[pidx((T - k) & (T - 1))];
-		float2 A = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
-		float2 B = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
-		stage[k * SP + 2 * grp] = A;
-		stage[k * SP + 2 * grp + 1] = B;
+	float2 sa[KP], sb[KP];
+#pragma unroll
+	for (int i = 0; i < KP; ++i) {
+		const int k = lane + i * FT;
+		if (k < CX) {
+			float2 z = fb[pidx(k)];
+			float2 zc = fb[pidx((T - k) & (T - 1))];
+			sa[i] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
+			sb[i] = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int i = 0; i < KP; ++i) {
+		const int k = lane + i * FT;
+		if (k < CX) {
+			stage[k * SP + 2 * grp] = sa[i];
+			stage[k * SP + 2 * grp + 1] = sb[i];
+		}
 	}
 	__syncthreads();
 	float2* dst = rt + blockIdx.y * rt_stride + (long long)t * CX * T + rb * ROWS;
