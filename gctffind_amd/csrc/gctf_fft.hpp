@@ -194,6 +194,157 @@ __device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ 
 	fft_run<N, 1, NT, INV>(buf, tw, tid);
 }
 
+// ---- compile-time transform with input and output hooks -----------------
+// fft_lds_io runs the same passes as fft_lds, but the first pass takes its
+// inputs from ld(i) (logical input index i) instead of buf, and the last
+// pass stores st(i, v) at logical output index i.  ld may read buf: every
+// read of a pass precedes its writes.  Pointwise work before and after a
+// transform (chirps, filters, crops) then costs no LDS round trip and no
+// barrier of its own.
+//
+// ZIN: the inputs at i >= N/2 are zero (ld is not called for them).  DROP:
+// the outputs at i >= N/2 are not needed (not stored).  In the first pass
+// element r of a butterfly has index j + r N/R with j < N/R, and in the
+// last pass j + r L with j < L = N/R, so both halves are r >= R/2: known
+// at compile time.
+
+// The plain input: buf itself.
+struct LdsIn {
+	const float2* buf;
+	__device__ __forceinline__ float2 operator()(int i) const { return buf[pidx(i)]; }
+};
+// The plain output: v unchanged.
+struct SameOut {
+	__device__ __forceinline__ float2 operator()(int, float2 v) const { return v; }
+};
+
+template <int N, int R, int L, int NT, bool INV, bool ZIN, bool DROP, class Ld, class St>
+__device__ __forceinline__ void fft_pass_io(float2* buf, const float2* __restrict__ tw, int tid,
+	const Ld& ld, const St& st)
+{
+	constexpr int T = N / R;
+	constexpr int BPT = (T + NT - 1) / NT;
+	float2 u[BPT][R];
+#pragma unroll
+	for (int b = 0; b < BPT; ++b) {
+		const int j = tid + b * NT;
+		if (T % NT == 0 || j < T) {
+#pragma unroll
+			for (int r = 0; r < R; ++r) {
+				if constexpr (L == 1) {
+					if (ZIN && 2 * r >= R) u[b][r] = make_float2(0.f, 0.f);
+					else u[b][r] = ld(j + r * T);
+				} else {
+					u[b][r] = buf[pidx(j + r * T)];
+				}
+			}
+			if constexpr (L > 1) {
+				const int k = j & (L - 1);
+				constexpr int S = N / (L * R);
+				float2 w = tw[k * S];
+				if (INV) w.y = -w.y;
+				twiddle_powers<R>(u[b], w);
+			}
+			dft_small<R, INV>(u[b]);
+		}
+	}
+	fft_sync<NT>();
+#pragma unroll
+	for (int b = 0; b < BPT; ++b) {
+		const int j = tid + b * NT;
+		if (T % NT == 0 || j < T) {
+			const int k = j & (L - 1);
+			const int base = (j - k) * R + k;
+#pragma unroll
+			for (int r = 0; r < R; ++r) {
+				if constexpr (L * R == N) {
+					if (!(DROP && 2 * r >= R)) buf[pidx(base + r * L)] = st(base + r * L, u[b][r]);
+				} else {
+					buf[pidx(base + r * L)] = u[b][r];
+				}
+			}
+		}
+	}
+	fft_sync<NT>();
+}
+
+template <int N, int L, int NT, bool INV, bool ZIN, bool DROP, class Ld, class St>
+__device__ __forceinline__ void fft_run_io(float2* buf, const float2* __restrict__ tw, int tid,
+	const Ld& ld, const St& st)
+{
+	if constexpr (L < N) {
+		constexpr int rem = N / L;
+		constexpr int R = rem >= 8 ? 8 : rem;
+		fft_pass_io<N, R, L, NT, INV, ZIN, DROP>(buf, tw, tid, ld, st);
+		fft_run_io<N, L * R, NT, INV, ZIN, DROP>(buf, tw, tid, ld, st);
+	}
+}
+
+// Same contract as fft_lds, except that the input need not be in buf: the
+// first pass reads ld(i).  N >= 16 (the first pass is not the last).
+template <int N, int NT, bool INV, bool ZIN = false, bool DROP = false, class Ld, class St>
+__device__ __forceinline__ void fft_lds_io(float2* buf, const float2* __restrict__ tw, int tid,
+	const Ld& ld, const St& st)
+{
+	static_assert(N >= 16, "the first and last passes must differ");
+	fft_run_io<N, 1, NT, INV, ZIN, DROP>(buf, tw, tid, ld, st);
+}
+
+// Bluestein DFT of length n on the compile-time working length M
+// (M >= 2n-1), input element i < n given by src(i) (which may read buf),
+// result in logical buf[0..n).  The chirp multiply and zero padding run in
+// the forward transform's first pass, the filter multiply in its last pass
+// and the final chirp multiply and 1/M scale in the inverse's last pass;
+// the arithmetic is that of bluestein_lds.
+// opaque(i) is i, but the compiler may not reuse an address or comparison
+// computed from the same index passes earlier: keeping those live across a
+// whole transform costs more registers than recomputing them.
+__device__ __forceinline__ int opaque(int i)
+{
+	asm volatile("" : "+v"(i));
+	return i;
+}
+
+template <int NT, int M, bool INV, class Src>
+__device__ __forceinline__ void bluestein_io(float2* buf, int n, const float2* __restrict__ chirp,
+	const float2* __restrict__ bhat, const float2* __restrict__ twm, int tid, const Src& src)
+{
+	// The hooks are branch-free (clamped index, then a select): branches
+	// inside a pass split the live ranges of the addresses the passes share.
+	auto ld = [&](int i) -> float2 {
+		const int ic = min(i, n - 1);
+		float2 w = chirp[ic];
+		if (INV) w.y = -w.y;
+		const float2 v = cmul(src(ic), w);
+		return i < n ? v : make_float2(0.f, 0.f);
+	};
+	auto filt = [&](int k, float2 v) -> float2 {
+		k = opaque(k);
+		float2 b;
+		if (INV) {
+			b = bhat[(M - k) & (M - 1)];
+			b.y = -b.y;
+		} else {
+			b = bhat[k];
+		}
+		return cmul(v, b);
+	};
+	// n <= M/2: the chirped input is zero from M/2 on, and only the first n
+	// outputs of the inverse are kept
+	fft_lds_io<M, NT, false, true, false>(buf, twm, tid, ld, filt);
+	const float inv_m = 1.0f / (float)M;
+	auto fin = [&](int k, float2 v) -> float2 {
+		k = opaque(k);
+		const int kc = min(k, n - 1);
+		float2 w = chirp[kc];
+		if (INV) w.y = -w.y;
+		const float2 t = cmul(v, w);
+		return k < n ? make_float2(t.x * inv_m, t.y * inv_m) : v;
+	};
+	// the inverse recomputes its addresses rather than keeping the forward's
+	fft_lds_io<M, NT, true, false, true>(buf, twm, opaque(tid), LdsIn{buf}, fin);
+}
+
 // ---- runtime-length power-of-two FFT (lengths up to MAXN) ----------------
 template <int R, int NT, int MAXN, bool INV>
 __device__ __forceinline__ void fft_pass_rt(float2* buf, const float2* __restrict__ tw, int tid,
@@ -316,7 +467,35 @@ template <int NT, int MAXN, bool INV>
 __device__ __forceinline__ void fft_any_lds(float2* buf, const Fft1dPlan& p, int tid)
 {
 	if (p.chirp == nullptr) fft_lds_rt<NT, MAXN, INV>(buf, p.n, p.tw, tid);
+	else if (p.m == MAXN) bluestein_io<NT, MAXN, INV>(buf, p.n, p.chirp, p.bhat, p.tw, tid, LdsIn{buf});
 	else bluestein_lds<NT, MAXN, INV>(buf, p.n, p.m, p.chirp, p.bhat, p.tw, tid);
+}
+
+// The same transform with the input element i < p.n given by src(i) instead
+// of buf (src may read buf).  Bluestein plans of working length MAXN read src
+// inside the first pass; other plans first gather src into buf.
+template <int NT, int MAXN, bool INV, class Src>
+__device__ __forceinline__ void fft_any_lds_src(float2* buf, const Fft1dPlan& p, int tid, const Src& src)
+{
+	if (p.chirp != nullptr && p.m == MAXN) {
+		bluestein_io<NT, MAXN, INV>(buf, p.n, p.chirp, p.bhat, p.tw, tid, src);
+		return;
+	}
+	constexpr int PER = (MAXN + NT - 1) / NT;
+	float2 v[PER];
+#pragma unroll
+	for (int i = 0; i < PER; ++i) {
+		const int y = tid + i * NT;
+		if (y < p.n) v[i] = src(y);
+	}
+	__syncthreads();
+#pragma unroll
+	for (int i = 0; i < PER; ++i) {
+		const int y = tid + i * NT;
+		if (y < p.n) buf[pidx(y)] = v[i];
+	}
+	__syncthreads();
+	fft_any_lds<NT, MAXN, INV>(buf, p, tid);
 }
 
 }  // namespace gctf

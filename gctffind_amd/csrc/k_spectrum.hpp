@@ -25,10 +25,15 @@ __device__ __forceinline__ int xcd_group(int b, int n)
 	return i < n ? i : -1;
 }
 
+// Waves per SIMD the rescale kernels are compiled for: their LDS buffer
+// (fft_lds_len(MAXN) complex values) leaves room for 32 waves per CU up to
+// MAXN = 8192.
+__host__ __device__ constexpr int rescale_waves(int maxn) { return maxn <= 8192 ? 8 : 4; }
+
 // Real rows -> half-spectrum rows kept for kx < ncx_out.  Two rows share one
 // complex transform (z = a + i b).  Grid (ceil(ny/2), images).
 template <typename Tin, int NT, int MAXN>
-__global__ __launch_bounds__(NT) void k_rescale_rows_r2c(const Tin* __restrict__ img,
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_r2c(const Tin* __restrict__ img,
 	long long img_stride, int nx, int ny, Fft1dPlan plan, int ncx_out,
 	float2* __restrict__ out, long long out_stride)
 {
@@ -81,7 +86,7 @@ __global__ __launch_bounds__(256) void k_transpose_c(const float2* __restrict__ 
 // inverse length nny, written back row-major: out[y'][kx] (ncx entries per
 // y').  Grid (8 ceil(ncx / 8), images).
 template <int NT, int MAXN>
-__global__ __launch_bounds__(NT) void k_rescale_cols(const float2* __restrict__ in,
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_cols(const float2* __restrict__ in,
 	long long in_stride, int ny, int nny, int ncx, Fft1dPlan pf, Fft1dPlan pi,
 	float2* __restrict__ out, long long out_stride)
 {
@@ -93,25 +98,12 @@ __global__ __launch_bounds__(NT) void k_rescale_cols(const float2* __restrict__ 
 	for (int y = tid; y < ny; y += NT) buf[pidx(y)] = src[y];
 	__syncthreads();
 	fft_any_lds<NT, MAXN, false>(buf, pf, tid);
-	// crop: gather into registers, then rewrite the head of the buffer
-	constexpr int PER = (MAXN + NT - 1) / NT;
-	float2 v[PER];
-#pragma unroll
-	for (int i = 0; i < PER; ++i) {
-		int y = tid + i * NT;
-		if (y < nny) {
-			int ys = y > nny / 2 ? y - nny + ny : y;
-			v[i] = buf[pidx(ys)];
-		}
-	}
-	__syncthreads();
-#pragma unroll
-	for (int i = 0; i < PER; ++i) {
-		int y = tid + i * NT;
-		if (y < nny) buf[pidx(y)] = v[i];
-	}
-	__syncthreads();
-	fft_any_lds<NT, MAXN, true>(buf, pi, tid);
+	// the crop is the inverse transform's input
+	auto crop = [&](int y) -> float2 {
+		const int ys = y > nny / 2 ? y - nny + ny : y;
+		return buf[pidx(ys)];
+	};
+	fft_any_lds_src<NT, MAXN, true>(buf, pi, tid, crop);
 	float2* dst = out + blockIdx.y * out_stride + kx;
 	for (int y = tid; y < nny; y += NT) dst[(long long)y * ncx] = buf[pidx(y)];
 }
@@ -121,7 +113,7 @@ __global__ __launch_bounds__(NT) void k_rescale_cols(const float2* __restrict__ 
 // Hermitian extension).  Two rows per transform.  Output rows have `pitch`
 // floats; the pad columns are zeroed.  Grid (ceil(nny/2), images).
 template <int NT, int MAXN>
-__global__ __launch_bounds__(NT) void k_rescale_rows_c2r(const float2* __restrict__ in,
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r(const float2* __restrict__ in,
 	long long in_stride, int nny, int nnx, Fft1dPlan plan, float* __restrict__ out,
 	long long out_stride, int pitch)
 {
@@ -132,7 +124,9 @@ __global__ __launch_bounds__(NT) void k_rescale_rows_c2r(const float2* __restric
 	const bool two = (y0 + 1) < nny;
 	const float2* ga = in + blockIdx.y * in_stride + (long long)y0 * ncx;
 	const float2* gb = ga + ncx;
-	for (int k = tid; k < nnx; k += NT) {
+	// Z = A + i B of the Hermitian extensions, read straight into the
+	// transform's first pass
+	auto herm = [&](int k) -> float2 {
 		float2 a, b;
 		if (k < ncx) {
 			a = ga[k];
@@ -143,11 +137,9 @@ __global__ __launch_bounds__(NT) void k_rescale_rows_c2r(const float2* __restric
 			b = two ? gb[nnx - k] : make_float2(0.f, 0.f);
 			b.y = -b.y;
 		}
-		// Z = A + i B
-		buf[pidx(k)] = make_float2(a.x - b.y, a.y + b.x);
-	}
-	__syncthreads();
-	fft_any_lds<NT, MAXN, true>(buf, plan, tid);
+		return make_float2(a.x - b.y, a.y + b.x);
+	};
+	fft_any_lds_src<NT, MAXN, true>(buf, plan, tid, herm);
 	float* da = out + blockIdx.y * out_stride + (long long)y0 * pitch;
 	for (int x = tid; x < pitch; x += NT) {
 		float2 z = x < nnx ? buf[pidx(x)] : make_float2(0.f, 0.f);
