@@ -38,7 +38,7 @@ VALU_PEAK_TFLOPS = 78.6   # FP32 FMA without packed math: 256 CU x 64 lanes x 2 
 # write x1.000); the table is profiles/pmc_traffic_plain_batch32_4096sq.md.
 MEASURED_TRAFFIC_PER_IMAGE = {
     "k_rescale_rows_r2c": 123.15e6, "k_rescale_cols": 103.25e6, "k_rescale_rows_c2r": 94.05e6,
-    "k_block_sums": 49.55e6, "k_tile_rows": 197.05e6, "k_tile_cols": 163.35e6,
+    "k_tile_rows": 197.05e6, "k_tile_cols": 163.35e6,
 }
 
 # Arithmetic of the FFT kernels per micrograph (5 N log2 N per complex
@@ -293,7 +293,7 @@ def kernel_bytes(name, g, per_launch_images, nsplit=8):
         b = g["ncx"] * g["ny"] * 8 + g["ncx"] * g["nny"] * 8
     elif name == "k_rescale_rows_c2r":
         b = g["nny"] * g["ncx"] * 8 + g["nny"] * g["pitch"] * 4
-    elif name == "k_block_sums":
+    elif name == "k_row_sums":
         b = g["cover"] * 4
     elif name == "k_tile_rows":
         b = g["cover"] * 4 + g["ntiles"] * g["cx"] * g["T"] * 8

@@ -177,7 +177,7 @@ struct gctf_ctx {
 
 	// batch buffers
 	int cap = 0;  // images the per-image buffers hold
-	DevBuf cbufA, cbufB, padc, bsums, rt, tpart;
+	DevBuf cbufA, cbufB, padc, bsums, rowsum, rt, tpart;
 	DevBuf raw, hsum, spect, mom, q, radial, ring_spec, ring_ss, states, part2d, refine, shell_cc, fullsp,
 		moments1, plans, shell_bins;
 	int kstride = 0;  // ring pixels reserved per image in ring_spec
@@ -590,9 +590,11 @@ int ensure_image_buffers(gctf_ctx* c, int n, int T, int ring_n)
 // ---------------------------------------------------------------------------
 
 // Rescale n images (CRescaleImage::DoIt) into padded rows at out + i * out_stride.
+// With tg, the C2R pass also leaves the row partials of tg's mean blocks in
+// c->rowsum (launch_spectrum then reads them instead of the image).
 template <typename Tin>
 int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long long in_stride, int n,
-	float* out, long long out_stride)
+	float* out, long long out_stride, const TileGrid* tg)
 {
 	hipStream_t s = c->stream;
 	const int ncx = rp.nnx / 2 + 1;
@@ -631,10 +633,12 @@ int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long lon
 		const int m = rp.c2r->m;
 		dim3 grid((rp.nny + 1) / 2, n);
 		Fft1dPlan p = rp.c2r->plan();
-		PROF(c, "k_rescale_rows_c2r", if (m <= 2048) k_rescale_rows_c2r<256, 2048><<<grid, 256, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
-			else if (m <= 4096) k_rescale_rows_c2r<512, 4096><<<grid, 512, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
-			else if (m <= 8192) k_rescale_rows_c2r<1024, 8192><<<grid, 1024, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch);
-			else k_rescale_rows_c2r<1024, 16384><<<grid, 1024, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch));
+		const TileGrid g = tg ? *tg : TileGrid{};
+		float* rs = tg ? c->rowsum.as<float>() : nullptr;
+		PROF(c, "k_rescale_rows_c2r", if (m <= 2048) k_rescale_rows_c2r<256, 2048><<<grid, 256, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs);
+			else if (m <= 4096) k_rescale_rows_c2r<512, 4096><<<grid, 512, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs);
+			else if (m <= 8192) k_rescale_rows_c2r<1024, 8192><<<grid, 1024, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs);
+			else k_rescale_rows_c2r<1024, 16384><<<grid, 1024, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs));
 	}
 	HIPCHK(c, hipGetLastError());
 	return GCTF_OK;
@@ -650,11 +654,11 @@ int ensure_rescale_buffers(gctf_ctx* c, const RescalePlan& rp, int n)
 }
 
 int launch_rescale(gctf_ctx* c, const RescalePlan& rp, const void* in, int dtype, long long in_stride,
-	int n, float* out, long long out_stride)
+	int n, float* out, long long out_stride, const TileGrid* tg = nullptr)
 {
 	if (dtype == GCTF_DTYPE_F16)
-		return launch_rescale_t<_Float16>(c, rp, (const _Float16*)in, in_stride, n, out, out_stride);
-	return launch_rescale_t<float>(c, rp, (const float*)in, in_stride, n, out, out_stride);
+		return launch_rescale_t<_Float16>(c, rp, (const _Float16*)in, in_stride, n, out, out_stride, tg);
+	return launch_rescale_t<float>(c, rp, (const float*)in, in_stride, n, out, out_stride, tg);
 }
 
 int ensure_spectrum_buffers(gctf_ctx* c, const TileGrid& g, int T, int n)
@@ -662,22 +666,32 @@ int ensure_spectrum_buffers(gctf_ctx* c, const TileGrid& g, int T, int n)
 	const size_t cx = T / 2 + 1;
 	const size_t ntiles = (size_t)g.ntx * g.nty;
 	CHK(ensure_buf(c, c->bsums, sizeof(double) * (size_t)g.nbx * g.nby * kSumParts * n));
+	const size_t nr = (size_t)(g.nby - 1) * g.step + g.bsize;   // rows of the mean blocks
+	CHK(ensure_buf(c, c->rowsum, sizeof(float) * nr * g.nbx * n));
 	CHK(ensure_buf(c, c->rt, sizeof(float2) * ntiles * cx * T * n));
 	CHK(ensure_buf(c, c->tpart, sizeof(float) * (size_t)c->tile_splits * T * cx * n));
 	return GCTF_OK;
 }
 
 // Tile-averaged amplitude spectra (CGenAvgSpectrum::DoIt) of n images into
-// raw[slot0 .. slot0+n).
+// raw[slot0 .. slot0+n).  rows_ready: the rescale left the mean blocks' row
+// partials in c->rowsum.
 template <typename Tin, int T>
 int launch_spectrum_t(gctf_ctx* c, const TileGrid& g, const TileTables& tt, const Tin* img, int pitch,
-	long long stride, int n, int slot0)
+	long long stride, int n, int slot0, bool rows_ready)
 {
 	hipStream_t s = c->stream;
 	constexpr int CX = T / 2 + 1;
 	const int ntiles = g.ntx * g.nty;
 	const int nsplit = std::min(c->tile_splits, ntiles);
-	PROF(c, "k_block_sums", k_block_sums<Tin><<<dim3(g.nbx * g.nby, kSumParts, n), 256, 0, s>>>(img, stride, pitch, g, c->bsums.as<double>()));
+	if (!rows_ready) {
+		const int nr = (g.nby - 1) * g.step + g.bsize;
+		PROF(c, "k_row_sums", k_row_sums<Tin><<<dim3(nr, n), 256, 0, s>>>(img, stride, pitch, g,
+			c->rowsum.as<float>()));
+	}
+	const int np = g.nbx * g.nby * kSumParts;
+	PROF(c, "k_block_rows", k_block_rows<<<dim3((np + 255) / 256, n), 256, 0, s>>>(c->rowsum.as<float>(), g,
+		c->bsums.as<double>()));
 	constexpr int ROWS = 16;
 	const long long rts = (long long)ntiles * CX * T;
 	PROF(c, "k_tile_rows", k_tile_rows<Tin, T, ROWS><<<dim3(ntiles * (T / ROWS), n), (ROWS / 2) * (T / 8), 0, s>>>(
@@ -696,12 +710,12 @@ int launch_spectrum_t(gctf_ctx* c, const TileGrid& g, const TileTables& tt, cons
 
 template <typename Tin>
 int launch_spectrum(gctf_ctx* c, const TileGrid& g, const TileTables& tt, const Tin* img, int pitch,
-	long long stride, int n, int slot0)
+	long long stride, int n, int slot0, bool rows_ready = false)
 {
 	switch (g.tile) {
-	case 256: return launch_spectrum_t<Tin, 256>(c, g, tt, img, pitch, stride, n, slot0);
-	case 512: return launch_spectrum_t<Tin, 512>(c, g, tt, img, pitch, stride, n, slot0);
-	case 1024: return launch_spectrum_t<Tin, 1024>(c, g, tt, img, pitch, stride, n, slot0);
+	case 256: return launch_spectrum_t<Tin, 256>(c, g, tt, img, pitch, stride, n, slot0, rows_ready);
+	case 512: return launch_spectrum_t<Tin, 512>(c, g, tt, img, pitch, stride, n, slot0, rows_ready);
+	case 1024: return launch_spectrum_t<Tin, 1024>(c, g, tt, img, pitch, stride, n, slot0, rows_ready);
 	default: return fail(c, GCTF_ERR_UNSUPPORTED, "tile size");
 	}
 }
@@ -1446,8 +1460,8 @@ int record_spectra(gctf_ctx* c, const RescalePlan& rp, TileTables& tt, const voi
 		if (rp.active) {
 			float* pad = c->padc.as<float>();
 			const long long ps = (long long)rp.pitch * rp.nny;
-			CHK(launch_rescale(c, rp, src, dtype, stride, m, pad, ps));
-			CHK(launch_spectrum<float>(c, g, tt, pad, rp.pitch, ps, m, i0));
+			CHK(launch_rescale(c, rp, src, dtype, stride, m, pad, ps, &g));
+			CHK(launch_spectrum<float>(c, g, tt, pad, rp.pitch, ps, m, i0, true));
 		} else if (dtype == GCTF_DTYPE_F16) {
 			CHK(launch_spectrum<_Float16>(c, g, tt, (const _Float16*)src, rp.nx, stride, m, i0));
 		} else {
@@ -2008,8 +2022,8 @@ int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result
 			if (rp.active) {
 				float* pad = ctx->padc.as<float>();
 				const long long ps = (long long)rp.pitch * rp.nny;
-				CHK(launch_rescale(ctx, rp, im.data, im.dtype, st, 1, pad, ps));
-				CHK(launch_spectrum<float>(ctx, it[i].g, *tt, pad, rp.pitch, ps, 1, i));
+				CHK(launch_rescale(ctx, rp, im.data, im.dtype, st, 1, pad, ps, &it[i].g));
+				CHK(launch_spectrum<float>(ctx, it[i].g, *tt, pad, rp.pitch, ps, 1, i, true));
 			} else if (im.dtype == GCTF_DTYPE_F16) {
 				CHK(launch_spectrum<_Float16>(ctx, it[i].g, *tt, (const _Float16*)im.data, im.nx, st, 1, i));
 			} else {

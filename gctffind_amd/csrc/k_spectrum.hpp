@@ -25,6 +25,22 @@ __device__ __forceinline__ int xcd_group(int b, int n)
 	return i < n ? i : -1;
 }
 
+struct TileGrid {
+	int ntx, nty, offx, offy, step, tile;
+	int blocks;          // 1: tile sums from 2x2 blocks of step x step
+	int nbx, nby;        // block counts
+	int bsize;           // block side (step, or tile when blocks == 0)
+};
+
+// Sum over a wavefront of the lanes' partial sums of a row segment (xor
+// butterfly, a fixed order); every lane returns the same value.
+__device__ __forceinline__ float wave_row_sum(float v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+	return v;
+}
+
 // Waves per SIMD the rescale kernels are compiled for: their LDS buffer
 // (fft_lds_len(MAXN) complex values) leaves room for 32 waves per CU up to
 // MAXN = 8192.
@@ -112,10 +128,15 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_cols(const 
 // DC and Nyquist bins are dropped, i.e. the real part of the inverse of the
 // Hermitian extension).  Two rows per transform.  Output rows have `pitch`
 // floats; the pad columns are zeroed.  Grid (ceil(nny/2), images).
+//
+// With rowsum, the row segments of the tile grid's mean blocks are summed
+// too (one wavefront per block column, FP32, fixed order):
+// rowsum[image][y - offy][bi] for the block rows y - offy in
+// [0, (nby-1) step + bsize).  k_block_rows turns them into block sums.
 template <int NT, int MAXN>
 __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r(const float2* __restrict__ in,
 	long long in_stride, int nny, int nnx, Fft1dPlan plan, float* __restrict__ out,
-	long long out_stride, int pitch)
+	long long out_stride, int pitch, TileGrid g, float* __restrict__ rowsum)
 {
 	__shared__ float2 buf[fft_lds_len(MAXN)];
 	const int tid = threadIdx.x;
@@ -146,6 +167,27 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r(co
 		da[x] = z.x;
 		if (two) da[pitch + x] = z.y;
 	}
+	if (rowsum == nullptr) return;
+	const int nr = (g.nby - 1) * g.step + g.bsize;
+	const int ra = y0 - g.offy;
+	if (ra + 1 < 0 || ra >= nr) return;
+	float* rs = rowsum + (long long)blockIdx.y * nr * g.nbx;
+	const int wave = tid >> 6, lane = tid & 63;
+	for (int bi = wave; bi < g.nbx; bi += NT / 64) {
+		const int x0 = g.offx + bi * g.step;
+		float sa = 0.f, sb = 0.f;
+		for (int i = lane; i < g.bsize; i += 64) {
+			const float2 z = buf[pidx(x0 + i)];
+			sa += z.x;
+			sb += z.y;
+		}
+		sa = wave_row_sum(sa);
+		sb = wave_row_sum(sb);
+		if (lane == 0) {
+			if (ra >= 0) rs[ra * g.nbx + bi] = sa;
+			if (two && ra + 1 < nr) rs[(ra + 1) * g.nbx + bi] = sb;
+		}
+	}
 }
 
 // ---------------------------------------------------------------------------
@@ -153,43 +195,55 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r(co
 // GRoundEdge.cu, GCalcSpectrum.cu:15-31,90-105, GAddImages.cu)
 // ---------------------------------------------------------------------------
 
-struct TileGrid {
-	int ntx, nty, offx, offy, step, tile;
-	int blocks;          // 1: tile sums from 2x2 blocks of step x step
-	int nbx, nby;        // block counts
-	int bsize;           // block side (step, or tile when blocks == 0)
-};
 
-// FP64 sums of bsize x bsize blocks at (offx + i*step, offy + j*step), each
-// block split over kSumParts row groups.  Grid (nbx*nby, kSumParts, images),
-// 256 threads; sums[image][block][part].
+// Tile means (CGenAvgSpectrum.cpp:84-139) come from FP64 sums of
+// bsize x bsize blocks at (offx + i*step, offy + j*step), each block split
+// into kSumParts row groups: sums[image][block][part].  They are built in
+// two steps.  First the row partials: for every block row y - offy in
+// [0, (nby-1) step + bsize) and block column bi, the FP32 sum of the row's
+// bsize pixels, summed by one wavefront (lane l adds pixels l, l+64, ...,
+// then an xor butterfly) into rowsum[image][y - offy][bi].  The rescale's
+// C2R pass makes them from its output rows in LDS; k_row_sums makes them
+// from an image in memory, in the same order, so both give the same bits.
 constexpr int kSumParts = 8;
 
+// Grid (nr, images), 256 threads: one block row per workgroup, one block
+// column per wavefront.
 template <typename Tin>
-__global__ __launch_bounds__(256) void k_block_sums(const Tin* __restrict__ img,
-	long long img_stride, int pitch, TileGrid g, double* __restrict__ sums)
+__global__ __launch_bounds__(256) void k_row_sums(const Tin* __restrict__ img, long long img_stride,
+	int pitch, TileGrid g, float* __restrict__ rowsum)
 {
-	__shared__ double red[256];
-	const int bi = blockIdx.x % g.nbx, bj = blockIdx.x / g.nbx;
+	const int r = blockIdx.x;
+	const int nr = (g.nby - 1) * g.step + g.bsize;
+	const Tin* src = img + blockIdx.y * img_stride + (long long)(g.offy + r) * pitch;
+	float* rs = rowsum + ((long long)blockIdx.y * nr + r) * g.nbx;
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	for (int bi = wave; bi < g.nbx; bi += 4) {
+		const int x0 = g.offx + bi * g.step;
+		float a = 0.f;
+		for (int i = lane; i < g.bsize; i += 64) a += load_px(src, x0 + i);
+		a = wave_row_sum(a);
+		if (lane == 0) rs[bi] = a;
+	}
+}
+
+// Second step: part p of block (bi, bj) adds its rows' partials in order,
+// in FP64.  Grid (ceil(nbx nby kSumParts / 256), images).
+__global__ __launch_bounds__(256) void k_block_rows(const float* __restrict__ rowsum, TileGrid g,
+	double* __restrict__ sums)
+{
+	const int nblk = g.nbx * g.nby;
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= nblk * kSumParts) return;
+	const int b = i / kSumParts, p = i % kSumParts;
+	const int bi = b % g.nbx, bj = b / g.nbx;
 	const int rows = (g.bsize + kSumParts - 1) / kSumParts;
-	const int y0 = blockIdx.y * rows, y1 = min(g.bsize, y0 + rows);
-	const int sx = g.offx + bi * g.step, sy = g.offy + bj * g.step;
-	const Tin* src = img + blockIdx.z * img_stride;
-	const int tid = threadIdx.x;
-	float a = 0.f;
-	const int n = (y1 - y0) * g.bsize;
-	for (int i = tid; i < n; i += 256) {
-		const int y = y0 + i / g.bsize, x = i % g.bsize;
-		a += load_px(src, (long long)(sy + y) * pitch + sx + x);
-	}
-	red[tid] = (double)a;
-	__syncthreads();
-	for (int s = 128; s > 0; s >>= 1) {
-		if (tid < s) red[tid] += red[tid + s];
-		__syncthreads();
-	}
-	if (tid == 0)
-		sums[((long long)blockIdx.z * g.nbx * g.nby + blockIdx.x) * kSumParts + blockIdx.y] = red[0];
+	const int r0 = bj * g.step + p * rows, r1 = bj * g.step + min(g.bsize, (p + 1) * rows);
+	const int nr = (g.nby - 1) * g.step + g.bsize;
+	const float* rs = rowsum + (long long)blockIdx.y * nr * g.nbx;
+	double acc = 0.0;
+	for (int r = r0; r < r1; ++r) acc += (double)rs[r * g.nbx + bi];
+	sums[((long long)blockIdx.y * nblk + b) * kSumParts + p] = acc;
 }
 
 __device__ __forceinline__ double block_sum(const double* bs, int b)
