@@ -37,8 +37,8 @@ VALU_PEAK_TFLOPS = 78.6   # FP32 FMA without packed math: 256 CU x 64 lanes x 2 
 # scaled by the same counters on a 256 MiB device copy (fetch x2.000,
 # write x1.000); the table is profiles/pmc_traffic_plain_batch32_4096sq.md.
 MEASURED_TRAFFIC_PER_IMAGE = {
-    "k_rescale_rows_r2c": 123.15e6, "k_rescale_cols": 103.25e6, "k_rescale_rows_c2r": 94.05e6,
-    "k_tile_rows": 197.05e6, "k_tile_cols": 163.35e6,
+    "k_rescale_rows_r2c": 123.08e6, "k_transpose_c": 138.41e6, "k_rescale_cols": 104.36e6,
+    "k_rescale_rows_c2r": 93.70e6, "k_tile_rows": 196.62e6, "k_tile_cols": 165.29e6,
 }
 
 # Arithmetic of the FFT kernels per micrograph (5 N log2 N per complex
