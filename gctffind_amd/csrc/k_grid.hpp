@@ -346,8 +346,7 @@ __global__ __launch_bounds__(256) void k_grid_rescore(GridSpec g, CtfConsts cc, 
 		for (int j = 0; j < CPB; ++j) {
 			float gg = fmaf(c2, cd[j].ca, s2 * cd[j].sa);
 			float df = fmaf(cd[j].sig, gg, cd[j].mean);
-			float u = fmaf(a, df, cd[j].ph0 - b);
-			float cv = (abs_sinpi(u) - 0.5f) * w;
+			float cv = sinpi_dev(fmaf(a, df, cd[j].ph0h - b)) * w;
 			acs[j] = fmaf(cv, sv, acs[j]);
 			acc2[j] = fmaf(cv, cv, acc2[j]);
 		}

@@ -16,21 +16,21 @@ constexpr float kD2R1D = 0.0174533f;     // CFindDefocus1D.cpp:11
 constexpr float kPi1D = 3.141592654f;    // GCalcCTF1D.cu:30
 constexpr float kPi2D = 3.1415926f;      // GCalcCTF2D.cu:35
 
-// |sin(pi u)|: u reduced to f in [0, 1), t = min(f, 1-f) in [0, 1/2],
-// sin(pi t) by its Taylor series to x^13 (|error| < 2e-9 on [0, pi/2]).
-__device__ __forceinline__ float abs_sinpi(float u)
+// |sin(pi u)| - 1/2 from v = u - 1/2: |sin(pi u)| = cos(pi r) with
+// r = v - rint(v) in [-1/2, 1/2] (exact), and cos(pi r) - 1/2 is its Taylor
+// series in r^2 to r^12 with the constant term folded in (truncation
+// < 7e-9; in FP32 with fused multiply-adds the result is within 2e-7 of
+// the exact value).
+__device__ __forceinline__ float sinpi_dev(float v)
 {
-	float f = u - floorf(u);
-	float t = fminf(f, 1.0f - f);
-	float x = t * 3.14159265358979f;
-	float x2 = x * x;
-	float p = fmaf(x2, -1.6059043836821614e-10f, 2.5052108385441720e-08f);
-	p = fmaf(x2, p, -2.7557319223985893e-06f);
-	p = fmaf(x2, p, 1.9841269841269841e-04f);
-	p = fmaf(x2, p, -8.3333333333333333e-03f);
-	p = fmaf(x2, p, 1.6666666666666667e-01f);
-	p = fmaf(-x2, p, 1.0f);
-	return x * p;
+	const float r = v - rintf(v);
+	const float z = r * r;
+	float p = fmaf(z, 1.9295743094039221e-03f, -2.5806891390014051e-02f);
+	p = fmaf(z, p, 2.3533063035889312e-01f);
+	p = fmaf(z, p, -1.3352627688545893e+00f);
+	p = fmaf(z, p, 4.0587121264167676e+00f);
+	p = fmaf(z, p, -4.9348022005446790e+00f);
+	return fmaf(z, p, 0.5f);
 }
 
 // ---------------------------------------------------------------------------
@@ -199,9 +199,10 @@ __device__ __forceinline__ bool stage_candidate(const SearchState& s, int c, flo
 
 // Per-candidate CTF parameters (CFindDefocus2D::mCorrelate :363-380 and
 // GCalcCTF2D::DoIt :99-114) in the form the evaluator uses:
-// u = chi / pi = ph0 + a * (mean + sig * g) - b with g = cos(2 (theta - az)).
+// u = chi / pi = ph0 + a * (mean + sig * g) - b with g = cos(2 (theta - az));
+// ph0h = ph0 - 1/2, the offset of v = u - 1/2 that sinpi_dev takes.
 struct CandCtf {
-	float mean, sig, ca, sa, ph0;
+	float mean, sig, ca, sa, ph0, ph0h;
 };
 
 __device__ __forceinline__ CandCtf cand_ctf(const CtfConsts& c, float dfm, float ast, float ang, float ph)
@@ -218,7 +219,47 @@ __device__ __forceinline__ CandCtf cand_ctf(const CtfConsts& c, float dfm, float
 	o.ca = (float)cos(a2);
 	o.sa = (float)sin(a2);
 	o.ph0 = (float)((double)add / 3.14159265358979323846);
+	o.ph0h = (float)((double)add / 3.14159265358979323846 - 0.5);
 	return o;
+}
+
+// Contribution of one ring pixel to CPB candidates' sums (c*S, c*c), with
+// c = (|sin(pi u)| - 1/2) w.  SHARED: the candidates have one azimuth and
+// one phase, so g and ph0h - b are formed once per pixel; the values are
+// those of the general form.
+template <int CPB, bool SHARED>
+__device__ __forceinline__ void cc2d_pixel(const CandCtf* cd, float a, float b, float c2, float s2,
+	float w, float sv, float* acc_cs, float* acc_cc)
+{
+	if constexpr (SHARED) {
+		const float g = fmaf(c2, cd[0].ca, s2 * cd[0].sa);
+		const float pb = cd[0].ph0h - b;
+#pragma unroll
+		for (int j = 0; j < CPB; ++j) {
+			const float df = fmaf(cd[j].sig, g, cd[j].mean);
+			const float cv = sinpi_dev(fmaf(a, df, pb)) * w;
+			acc_cs[j] = fmaf(cv, sv, acc_cs[j]);
+			acc_cc[j] = fmaf(cv, cv, acc_cc[j]);
+		}
+	} else {
+#pragma unroll
+		for (int j = 0; j < CPB; ++j) {
+			const float g = fmaf(c2, cd[j].ca, s2 * cd[j].sa);
+			const float df = fmaf(cd[j].sig, g, cd[j].mean);
+			const float cv = sinpi_dev(fmaf(a, df, cd[j].ph0h - b)) * w;
+			acc_cs[j] = fmaf(cv, sv, acc_cs[j]);
+			acc_cc[j] = fmaf(cv, cv, acc_cc[j]);
+		}
+	}
+}
+
+template <int CPB, bool SHARED>
+__device__ __forceinline__ void cc2d_pixels(const CandCtf* cd, const RingTables& ring, const float* sp,
+	int k0, int k1, float* acc_cs, float* acc_cc)
+{
+	for (int k = k0 + threadIdx.x; k < k1; k += 256)
+		cc2d_pixel<CPB, SHARED>(cd, ring.a[k], ring.b[k], ring.c2[k], ring.s2[k], ring.w[k], sp[k], acc_cs,
+			acc_cc);
 }
 
 // Candidate evaluation.  Grid (ceil(max candidates / CPB), kPixChunks,
@@ -243,7 +284,7 @@ __global__ __launch_bounds__(256) void k_cc2d_eval(const ImgPlan* __restrict__ p
 		float dfm, ast, ang, ph;
 		bool ok = c < n && stage_candidate(S, c, dfm, ast, ang, ph);
 		cvalid[threadIdx.x] = ok;
-		cand[threadIdx.x] = ok ? cand_ctf(cc, dfm, ast, ang, ph) : CandCtf{0.f, 0.f, 0.f, 0.f, 0.f};
+		cand[threadIdx.x] = ok ? cand_ctf(cc, dfm, ast, ang, ph) : CandCtf{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 	}
 	__syncthreads();
 	const int chunk = blockIdx.y;
@@ -254,21 +295,16 @@ __global__ __launch_bounds__(256) void k_cc2d_eval(const ImgPlan* __restrict__ p
 #pragma unroll
 	for (int j = 0; j < CPB; ++j) { acc_cs[j] = 0.f; acc_cc[j] = 0.f; }
 	CandCtf cd[CPB];
+	bool shared = true;
 #pragma unroll
-	for (int j = 0; j < CPB; ++j) cd[j] = cand[j];
-	for (int k = k0 + threadIdx.x; k < k1; k += 256) {
-		const float a = ring.a[k], b = ring.b[k], c2 = ring.c2[k], s2 = ring.s2[k];
-		const float w = ring.w[k], sv = sp[k];
-#pragma unroll
-		for (int j = 0; j < CPB; ++j) {
-			float g = fmaf(c2, cd[j].ca, s2 * cd[j].sa);
-			float df = fmaf(cd[j].sig, g, cd[j].mean);
-			float u = fmaf(a, df, cd[j].ph0 - b);
-			float cv = (abs_sinpi(u) - 0.5f) * w;
-			acc_cs[j] = fmaf(cv, sv, acc_cs[j]);
-			acc_cc[j] = fmaf(cv, cv, acc_cc[j]);
-		}
+	for (int j = 0; j < CPB; ++j) {
+		cd[j] = cand[j];
+		// invalid candidates are zero and their sums are not stored
+		if (c0 + j < n && cvalid[j])
+			shared = shared && cd[j].ca == cand[0].ca && cd[j].sa == cand[0].sa && cd[j].ph0h == cand[0].ph0h;
 	}
+	if (shared && cvalid[0]) cc2d_pixels<CPB, true>(cd, ring, sp, k0, k1, acc_cs, acc_cc);
+	else cc2d_pixels<CPB, false>(cd, ring, sp, k0, k1, acc_cs, acc_cc);
 	// reduce: wave shuffles in double, then across the 4 waves in LDS
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
