@@ -253,13 +253,25 @@ __device__ __forceinline__ void cc2d_pixel(const CandCtf* cd, float a, float b, 
 	}
 }
 
+// A pointer read from memory (here: the ring tables of a plan) is generic to
+// the compiler, whose loads through it are then flat loads, with 64-bit
+// address arithmetic and waits on the LDS counter too.  The tables are
+// device memory: gload reads them as global memory, from a uniform base and
+// a 32-bit index.
+typedef const float __attribute__((address_space(1)))* gfloat_ptr;
+__device__ __forceinline__ float gload(const float* p, int i) { return ((gfloat_ptr)p)[i]; }
+
+// The pixel loop is unrolled so that the loads of four pixels are in flight
+// together: the small stages run few waves per SIMD, which cannot hide the
+// latency of one pixel's loads behind another wave's arithmetic.
 template <int CPB, bool SHARED>
 __device__ __forceinline__ void cc2d_pixels(const CandCtf* cd, const RingTables& ring, const float* sp,
 	int k0, int k1, float* acc_cs, float* acc_cc)
 {
-	for (int k = k0 + threadIdx.x; k < k1; k += 256)
-		cc2d_pixel<CPB, SHARED>(cd, ring.a[k], ring.b[k], ring.c2[k], ring.s2[k], ring.w[k], sp[k], acc_cs,
-			acc_cc);
+#pragma unroll 4
+	for (int k = k0 + (int)threadIdx.x; k < k1; k += 256)
+		cc2d_pixel<CPB, SHARED>(cd, gload(ring.a, k), gload(ring.b, k), gload(ring.c2, k), gload(ring.s2, k),
+			gload(ring.w, k), gload(sp, k), acc_cs, acc_cc);
 }
 
 // Candidate evaluation.  Grid (ceil(max candidates / CPB), kPixChunks,
