@@ -123,6 +123,26 @@ def test_search_1d_2d_matches_oracle_on_same_spectrum():
     assert rg["ctf_res"] == pytest.approx(ro["ctf_res"], rel=1e-6)
 
 
+@pytest.mark.parametrize("seed,df1,df2,az", [(31, 9000.0, 8600.0, 5.0), (32, 26000.0, 25000.0, -75.0),
+                                              (33, 15000.0, 14900.0, 33.0), (35, 33000.0, 32000.0, 0.0)])
+def test_search_result_identical_to_oracle(seed, df1, df2, az):
+    """On the same spectrum the float32 search lands on the oracle's candidate:
+    defocus and azimuth equal, the CC within one float32 step (the FP32
+    per-thread sums against the oracle's FP64 ones)."""
+    img = _img(1024, 1.2, seed=seed, df1=df1, df2=df2, az_deg=az)
+    ctx, pad, (nnx, nny, pixn) = _gpu_spectra(img, 1.2)
+    sp_g = ctx.get_half_spect(raw=False)
+    fc = O.FindCtf2D(pix=pixn)
+    fc.set_half_spect(sp_g.astype(np.float64))
+    fc.set_phase(0.0, 0.0)
+    ro = fc.do2d()
+    ctx.set_phase(0.0, 0.0)
+    rg = ctx.do2d()
+    assert rg["df_min"] == ro["df_min"] and rg["df_max"] == ro["df_max"]
+    assert rg["azimuth"] == ro["azimuth"]
+    assert abs(rg["score"] - ro["score"]) <= 1e-7
+
+
 def test_refine_matches_oracle():
     img = _img(1024, 1.2, seed=7, df1=12500.0, df2=12000.0, az_deg=60.0)
     ctx, pad, (nnx, nny, pixn) = _gpu_spectra(img, 1.2)
