@@ -692,9 +692,11 @@ int launch_spectrum_t(gctf_ctx* c, const TileGrid& g, const TileTables& tt, cons
 	const int np = g.nbx * g.nby * kSumParts;
 	PROF(c, "k_block_rows", k_block_rows<<<dim3((np + 255) / 256, n), 256, 0, s>>>(c->rowsum.as<float>(), g,
 		c->bsums.as<double>()));
-	constexpr int ROWS = T >= 1024 ? 16 : 32;               // rows per row-pass workgroup (<= 1024 threads)
+	constexpr int ROWS = 16;                                 // rows per row-pass workgroup
 	const long long rts = (long long)ntiles * CX * T;
-	PROF(c, "k_tile_rows", k_tile_rows<Tin, T, ROWS><<<dim3(ntiles * (T / ROWS), n), (ROWS / 2) * (T / 8), 0, s>>>(
+	constexpr int RG = 2;                                    // row blocks per row-pass workgroup
+	PROF(c, "k_tile_rows", k_tile_rows<Tin, T, ROWS, RG><<<dim3((ntiles * (T / ROWS) + RG - 1) / RG, n),
+		(ROWS / 2) * (T / 8), 0, s>>>(
 		img, stride, pitch, g, c->bsums.as<double>(), tt.mask.as<float>(), tt.tw.as<float2>(),
 		c->rt.as<float2>(), rts));
 	constexpr int CPW = 4;

@@ -257,11 +257,14 @@ __device__ __forceinline__ double block_sum(const double* bs, int b)
 // Row pass of the tile FFT: for ROWS rows of one tile, subtract the tile
 // mean, apply the round-edge mask, R2C transform along x (two rows per
 // complex transform) and write the half-spectrum transposed:
-// rt[image][tile][kx][y].  Grid (ntiles * T/ROWS, images), (ROWS/2)*(T/8)
-// threads.  The transposing stage reuses the transforms' LDS (the split
-// halves wait in registers across a barrier), so four workgroups fit a CU.
-template <typename Tin, int T, int ROWS>
-__global__ __launch_bounds__((ROWS / 2) * (T / 8), 8) void k_tile_rows(const Tin* __restrict__ img,
+// rt[image][tile][kx][y].  A workgroup of (ROWS/2)*(T/8) threads takes G
+// consecutive row blocks (block b = rows [(b % (T/ROWS)) ROWS, +ROWS) of
+// tile b / (T/ROWS)); the image rows of the next block are loaded into
+// registers while the current one is transformed.  Grid
+// (ceil(ntiles * T/ROWS / G), images).  The transposing stage reuses the
+// transforms' LDS (the split halves wait in registers across a barrier).
+template <typename Tin, int T, int ROWS, int G>
+__global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_rows(const Tin* __restrict__ img,
 	long long img_stride, int pitch, TileGrid g, const double* __restrict__ sums,
 	const float* __restrict__ mask, const float2* __restrict__ tw,
 	float2* __restrict__ rt, long long rt_stride)
@@ -271,63 +274,86 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8), 8) void k_tile_rows(const Tin
 	constexpr int NT = NG * FT;
 	constexpr int CX = T / 2 + 1;
 	constexpr int FL = fft_lds_len(T);
+	constexpr int PX = T / FT;             // pixels per thread and row
 	constexpr int KP = (CX + FT - 1) / FT; // half-spectrum entries per thread
 	constexpr int SP = ROWS + 1;           // stage pitch: lanes of a wave write
 	                                       // k-consecutive entries, an odd pitch
 	                                       // keeps them off one LDS bank
+	constexpr int NRB = T / ROWS;          // row blocks per tile
 	static_assert(CX * SP <= NG * FL, "stage must fit in the transform buffers");
 	__shared__ float2 buf[NG * FL];
 	float2* stage = buf;
-	const int rb = blockIdx.x % (T / ROWS);
-	const int t = blockIdx.x / (T / ROWS);
-	const int tx = t % g.ntx, ty = t / g.ntx;
-	const int sx = g.offx + tx * g.step, sy = g.offy + ty * g.step;
+	const int nblk = g.ntx * g.nty * NRB;
+	const int grp = threadIdx.x / FT, lane = threadIdx.x % FT;
+	const Tin* im = img + blockIdx.y * img_stride;
 	const long long nb = (long long)g.nbx * g.nby * kSumParts;
 	const double* bs = sums + blockIdx.y * nb;
-	double tsum;
-	if (g.blocks) {
-		tsum = block_sum(bs, ty * g.nbx + tx) + block_sum(bs, ty * g.nbx + tx + 1)
-			+ block_sum(bs, (ty + 1) * g.nbx + tx) + block_sum(bs, (ty + 1) * g.nbx + tx + 1);
-	} else {
-		tsum = block_sum(bs, ty * g.nbx + tx);
-	}
-	const float mean = (float)(tsum / ((double)T * (double)T));
-	const int grp = threadIdx.x / FT, lane = threadIdx.x % FT;
-	const int ya = rb * ROWS + 2 * grp;
-	const Tin* src = img + blockIdx.y * img_stride + (long long)(sy + ya) * pitch + sx;
 	float2* fb = buf + grp * FL;
-	for (int x = lane; x < T; x += FT) {
-		float a = (load_px(src, x) - mean) * mask[ya * T + x];
-		float b = (load_px(src, (long long)pitch + x) - mean) * mask[(ya + 1) * T + x];
-		fb[pidx(x)] = make_float2(a, b);
-	}
-	fft_sync<FT>();
-	fft_lds<T, FT, false>(fb, tw, lane);
-	float2 sa[KP], sb[KP];
+	// this thread's pixels of the two rows of row block b
+	float va[PX], vb[PX];
+	auto load_rows = [&](int b) {
+		const int t = b / NRB, ya = (b % NRB) * ROWS + 2 * grp;
+		const int sx = g.offx + (t % g.ntx) * g.step, sy = g.offy + (t / g.ntx) * g.step;
+		const Tin* src = im + (long long)(sy + ya) * pitch + sx;
 #pragma unroll
-	for (int i = 0; i < KP; ++i) {
-		const int k = lane + i * FT;
-		if (k < CX) {
-			float2 z = fb[pidx(k)];
-			float2 zc = fb[pidx((T - k) & (T - 1))];
-			sa[i] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
-			sb[i] = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+		for (int i = 0; i < PX; ++i) {
+			va[i] = load_px(src, lane + i * FT);
+			vb[i] = load_px(src, (long long)pitch + lane + i * FT);
 		}
-	}
-	__syncthreads();
+	};
+	const int b0 = blockIdx.x * G;
+	if (b0 >= nblk) return;
+	load_rows(b0);
+	for (int it = 0; it < G; ++it) {
+		const int b = b0 + it;
+		if (b >= nblk) break;
+		const int rb = b % NRB, t = b / NRB;
+		const int tx = t % g.ntx, ty = t / g.ntx;
+		double tsum;
+		if (g.blocks) {
+			tsum = block_sum(bs, ty * g.nbx + tx) + block_sum(bs, ty * g.nbx + tx + 1)
+				+ block_sum(bs, (ty + 1) * g.nbx + tx) + block_sum(bs, (ty + 1) * g.nbx + tx + 1);
+		} else {
+			tsum = block_sum(bs, ty * g.nbx + tx);
+		}
+		const float mean = (float)(tsum / ((double)T * (double)T));
+		const int ya = rb * ROWS + 2 * grp;
 #pragma unroll
-	for (int i = 0; i < KP; ++i) {
-		const int k = lane + i * FT;
-		if (k < CX) {
-			stage[k * SP + 2 * grp] = sa[i];
-			stage[k * SP + 2 * grp + 1] = sb[i];
+		for (int i = 0; i < PX; ++i) {
+			const int x = lane + i * FT;
+			fb[pidx(x)] = make_float2((va[i] - mean) * mask[ya * T + x], (vb[i] - mean) * mask[(ya + 1) * T + x]);
 		}
-	}
-	__syncthreads();
-	float2* dst = rt + blockIdx.y * rt_stride + (long long)t * CX * T + rb * ROWS;
-	for (int i = threadIdx.x; i < CX * ROWS; i += NT) {
-		int k = i / ROWS, yy = i % ROWS;
-		dst[(long long)k * T + yy] = stage[k * SP + yy];
+		fft_sync<FT>();
+		if (it + 1 < G && b + 1 < nblk) load_rows(b + 1);
+		fft_lds<T, FT, false>(fb, tw, lane);
+		float2 sa[KP], sb[KP];
+#pragma unroll
+		for (int i = 0; i < KP; ++i) {
+			const int k = lane + i * FT;
+			if (k < CX) {
+				float2 z = fb[pidx(k)];
+				float2 zc = fb[pidx((T - k) & (T - 1))];
+				sa[i] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
+				sb[i] = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+			}
+		}
+		__syncthreads();
+#pragma unroll
+		for (int i = 0; i < KP; ++i) {
+			const int k = lane + i * FT;
+			if (k < CX) {
+				stage[k * SP + 2 * grp] = sa[i];
+				stage[k * SP + 2 * grp + 1] = sb[i];
+			}
+		}
+		__syncthreads();
+		float2* dst = rt + blockIdx.y * rt_stride + (long long)t * CX * T + rb * ROWS;
+		for (int i = threadIdx.x; i < CX * ROWS; i += NT) {
+			int k = i / ROWS, yy = i % ROWS;
+			dst[(long long)k * T + yy] = stage[k * SP + yy];
+		}
+		// the stage is the transforms' buffer: read out before it is refilled
+		__syncthreads();
 	}
 }
 
