@@ -1815,10 +1815,10 @@ int gctf_estimate_batch_learned(gctf_ctx* ctx, const void* dev_imgs, int dtype, 
 int gctf_learned_program_default(gctf_search_program* out)
 {
 	if (!out) return GCTF_ERR_ARG;
-	out->n_ast = 17; out->n_ang = 26;
+	out->n_ast = 11; out->n_ang = 18;
 	out->n_mag = 11; out->n_angl = 27;
 	out->n_df = 11;
-	out->iters = 10;
+	out->iters = 6;
 	out->df_minstep = 20.0f;
 	out->fine_n = 9;
 	out->fine_step = 5.0f;
