@@ -222,8 +222,8 @@ def timed(torch, run, n, steps, warmup):
 
 
 def evaluate(torch, ctx, a, buf):
-    """Learned path against the plain path (the reference algorithm, bitwise
-    the FP64 oracle's on the parity cases) on a.eval held-out micrographs,
+    """Learned path against the plain path (the reference algorithm; its
+    defocus and azimuth equal the FP64 oracle's on the parity cases) on a.eval held-out micrographs,
     seeds 2000..: mean-defocus agreement overall, inside the reference's 1D
     search range (3000 p^2 .. 30000 p^2 at the working pixel size p; outside
     it the reference itself fails) and by defocus band, agreement within the
@@ -252,7 +252,7 @@ def evaluate(torch, ctx, a, buf):
     dis = inr & (dd > 20.0)
     r = lambda x: round(float(x), 4)
     return {"n": int(t.size), "seeds": f"2000..{2000 + t.size - 1}",
-            "reference": "plain GPU path (the reference algorithm, bitwise equal to the FP64 oracle on the parity cases)",
+            "reference": "plain GPU path (the reference algorithm; defocus and azimuth equal to the FP64 oracle's on the parity cases)",
             "frac_within_20A_all": r(np.mean(dd <= 20.0)),
             "reference_1d_range_A": [round(lo1), round(hi1)],
             "frac_in_reference_range": r(np.mean(inr)),
