@@ -41,6 +41,16 @@ MEASURED_TRAFFIC_PER_IMAGE = {
     "k_rescale_rows_c2r": 93.70e6, "k_tile_rows": 196.62e6, "k_tile_cols": 165.29e6,
 }
 
+# Wave64 VALU instructions per 4096x4096 micrograph at 1.0 A (rocprofv3 --pmc
+# SQ_INSTS_VALU over `bench.py --mode plain`, all XCDs; the table is
+# profiles/pmc_valu_plain_batch32_4096sq.md), and the issue peak they are
+# measured against: one wave64 FP32 instruction per CU per cycle (a
+# microbenchmark of independent FMA/add/multiply chains reaches 0.97).
+MEASURED_VALU_INSTR_PER_IMAGE = {
+    "k_rescale_rows_r2c": 9.286e6, "k_rescale_cols": 40.531e6, "k_rescale_rows_c2r": 35.797e6,
+}
+VALU_ISSUE_PEAK = 256 * 2.4e9          # wave64 instructions per second
+
 # Arithmetic of the FFT kernels per micrograph (5 N log2 N per complex
 # transform of length N, 6 per complex multiply): k_rescale_cols runs 1707
 # columns of FFT 4096 + Bluestein 3412 (two FFTs of 8192, three pointwise
@@ -354,6 +364,14 @@ def roofline(ctx, ptr, n, a, run, tuned=None):
         roof["valu"] = {"achieved": round(tf, 1), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": round(tf / VALU_PEAK_TFLOPS, 4), "flop_per_launch": int(fl * per),
                         "note": "FFT-bound: 5 N log2 N per complex transform"}
+    vi = MEASURED_VALU_INSTR_PER_IMAGE.get(base) if std else None
+    if vi:
+        rate = vi * per / (rows[dom]["avg_us"] * 1e-6)
+        roof["valu_issue"] = {"achieved": round(rate / 1e9, 1), "peak": round(VALU_ISSUE_PEAK / 1e9, 1),
+                              "unit": "G wave64 instructions/s", "frac": round(rate / VALU_ISSUE_PEAK, 4),
+                              "instructions_per_launch": int(vi * per),
+                              "note": "SQ_INSTS_VALU per launch (PMC) over the live launch time; "
+                                      "peak one instruction per CU per cycle at 2.4 GHz"}
     return roof, rows
 
 
