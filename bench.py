@@ -80,6 +80,9 @@ def parse():
                     help="skip gctf_autotune (images per chunk x tile splits chosen by timing)")
     ap.add_argument("--chunk", type=int, default=0, help="images per spectrum chunk (0: library default)")
     ap.add_argument("--splits", type=int, default=0, help="tile-column splits (0: library default)")
+    ap.add_argument("--scan1d", default="",
+                    help="initial 1D defocus scan lo,hi,step in A for the GPU and the oracle "
+                         "(default: the reference's 3000 p^2 .. 30000 p^2 in 500 steps)")
     ap.add_argument("--no-plain-leg", action="store_true",
                     help="learned mode: do not time the plain path beside it (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -142,6 +145,9 @@ def main():
 
     ctx = gctf.Context()
     ctx.tune(chunk=a.chunk, tile_splits=a.splits)
+    scan1d = tuple(float(v) for v in a.scan1d.split(",")) if a.scan1d else None
+    if scan1d:
+        ctx.set_search1d(*scan1d)
     n = a.batch
     learned = a.mode == "learned"
     if learned:
@@ -187,6 +193,10 @@ def main():
             "batch": n, "size": a.size, "pixel_A": a.pix, "tile": 512,
             "path": a.mode,
             "autotune": tuned,
+            "search1d_scan": ({"df_lo_A": scan1d[0], "df_hi_A": scan1d[1], "step_A": scan1d[2],
+                               "points": int((scan1d[1] - scan1d[0]) / scan1d[2]) // 2 * 2 + 1}
+                              if scan1d else "reference: 3000 p^2 .. 30000 p^2 A in 500 steps "
+                                             "(4323-43234 A, 501 points at the working 1.20047 A)"),
         },
     }
     if learned:
@@ -212,7 +222,7 @@ def main():
     if not a.no_cpu_baseline:
         if learned and pres is None:
             pres = ctx.estimate_batch(ptr, n, a.size, a.size, a.pix)
-        out["cpu_baseline"], par = cpu_baseline(img0, a.pix, pres[0] if learned else res[0])
+        out["cpu_baseline"], par = cpu_baseline(img0, a.pix, pres[0] if learned else res[0], scan1d)
         out["config"]["parity_vs_oracle_img0"] = par
     print(json.dumps(out), flush=True)
 
@@ -391,11 +401,12 @@ def ctx_grid_geometry(ctx, a):
     return {"m": 256 * 32 * 36, "k": k}
 
 
-def cpu_baseline(img, pix, gpu_res):
-    """The CPU oracle (one core) on one micrograph of the same workload."""
+def cpu_baseline(img, pix, gpu_res, scan1d=None):
+    """The CPU oracle (one core) on one micrograph of the same workload, with
+    the same 1D scan as the GPU run."""
     import oracle as O
     t0 = time.perf_counter()
-    ref, _ = O.estimate(img, pix)
+    ref, _ = O.estimate(img, pix, scan1d=scan1d)
     dt = time.perf_counter() - t0
     par = {k: round(abs(gpu_res[k] - ref[k]), 4) for k in ("df_min", "df_max", "azimuth", "score")}
     return ({"value": round(1.0 / dt, 5), "unit": "micrographs/s", "cores": 1, "kind": "port",

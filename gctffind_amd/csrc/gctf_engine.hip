@@ -159,6 +159,7 @@ struct gctf_ctx {
 	gctf_params prm{};
 	std::string err;
 	int chunk = 4, tile_splits = 8, use_graph = 1;
+	float scan_lo = 0, scan_hi = 0, scan_step = 0;  // Do1D scan (gctf_set_search1d; 0: reference)
 	int gen = 0;  // bumped whenever a device buffer moves
 
 	std::map<int, std::unique_ptr<FftDev>> ffts;
@@ -792,6 +793,7 @@ int launch_1d(gctf_ctx* c, const TileTables& tt, int i0, int m, int mode, float 
 	Search1DParams p{};
 	p.cmpx = cx;
 	p.mode = mode; p.init_df = init_df; p.range = range;
+	p.scan_lo = c->scan_lo; p.scan_hi = c->scan_hi; p.scan_step = c->scan_step;
 	PROF(c, "k_search1d", k_search1d<<<m, 256, 0, s>>>(p, c->plans.as<ImgPlan>() + i0, radial,
 		tt.w1d.as<double>(), c->states.as<SearchState>() + i0));
 	HIPCHK(c, hipGetLastError());
@@ -1259,6 +1261,19 @@ int gctf_setup2(gctf_ctx* ctx, int nx, int ny)
 	if (nx < T || ny < T) return fail(ctx, GCTF_ERR_ARG, "image smaller than one tile");
 	ctx->s_nx = nx; ctx->s_ny = ny;
 	ctx->setup2 = true;
+	return GCTF_OK;
+}
+
+int gctf_set_search1d(gctf_ctx* ctx, float df_lo_A, float df_hi_A, float step_A)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	const bool dflt = df_lo_A == 0.0f && df_hi_A == 0.0f && step_A == 0.0f;
+	if (!dflt && !(df_lo_A > 0.0f && df_hi_A > df_lo_A && step_A > 0.0f))
+		return fail(ctx, GCTF_ERR_ARG, "1D scan: need 0 < df_lo < df_hi and step > 0 (or all 0)");
+	if (!dflt && (df_hi_A - df_lo_A) / step_A > 1e6f)
+		return fail(ctx, GCTF_ERR_ARG, "1D scan: more than 1e6 points");
+	ctx->scan_lo = df_lo_A; ctx->scan_hi = df_hi_A; ctx->scan_step = step_A;
+	ctx->gen++;   // recorded graphs hold the old scan
 	return GCTF_OK;
 }
 

@@ -44,6 +44,10 @@ struct Search1DParams {
 	float lo, hi;            // bins x with lo <= x < hi (Fourier pixels)
 	int mode;                // 0: Do1D, 1: Refine1D(init_df, range)
 	float init_df, range;    // Refine1D arguments
+	// initial Do1D scan (mFindDefocus): defocus [scan_lo, scan_hi] A in
+	// steps of scan_step A; 0 keeps the reference's 3000 p^2 .. 30000 p^2
+	// in 500 steps (CFindCtf1D.cpp:79-80, CFindDefocus1D.cpp:76-80)
+	float scan_lo, scan_hi, scan_step;
 };
 
 __device__ __forceinline__ float cc1d_eval(const Search1DParams& p, const float* rad,
@@ -68,13 +72,16 @@ __device__ __forceinline__ float cc1d_eval(const Search1DParams& p, const float*
 }
 
 // Brute force over (df, phase); returns the first strict maximum.
+// step > 0 replaces the reference's step (range / 500, at least 10 A); the
+// point count follows its rule either way.
 __device__ void search1d_brute(const Search1DParams& p, const float* rad, const double* w1d,
-	float dfr0, float dfr1, float phr0, float phr1, float* out3, float* s_cc, int* s_i)
+	float dfr0, float dfr1, float phr0, float phr1, float step, float* out3, float* s_cc, int* s_i)
 {
 	int ndf = 501;
 	float dfrange = dfr1 - dfr0;
 	float dfstep = dfrange / (float)(ndf - 1);
 	if (dfstep < 10) dfstep = 10.0f;
+	if (step > 0.0f) dfstep = step;
 	ndf = (int)(dfrange / dfstep) / 2 * 2 + 1;
 	int nps = 37;
 	float psrange = phr1 - phr0;
@@ -142,10 +149,11 @@ __global__ __launch_bounds__(256) void k_search1d(Search1DParams p0, const ImgPl
 	if (p.mode == 0) {
 		// CFindCtf1D::mFindDefocus
 		float p2 = pix * pix;
-		float dfr0 = 3000.0f * p2, dfr1 = 30000.0f * p2;
+		float dfr0 = p.scan_lo > 0.0f ? p.scan_lo : 3000.0f * p2;
+		float dfr1 = p.scan_hi > 0.0f ? p.scan_hi : 30000.0f * p2;
 		float phr0 = fmaxf(S->ext_phase - S->phase_range / 2, 0.0f);
 		float phr1 = fminf(phr0 + S->phase_range, 150.0f);
-		search1d_brute(p, rad, w1d, dfr0, dfr1, phr0, phr1, res, s_cc, s_i);
+		search1d_brute(p, rad, w1d, dfr0, dfr1, phr0, phr1, p.scan_step, res, s_cc, s_i);
 		df_min = res[0];
 		ext_phase = res[1];
 	} else {
@@ -161,7 +169,7 @@ __global__ __launch_bounds__(256) void k_search1d(Search1DParams p0, const ImgPl
 	float prange = 0.2f * S->phase_range;
 	float phr0 = fmaxf(ext_phase - prange / 2, 0.0f);
 	float phr1 = fminf(0.0f + prange, 150.0f);
-	search1d_brute(p, rad, w1d, dfr0, dfr1, phr0, phr1, res, s_cc, s_i);
+	search1d_brute(p, rad, w1d, dfr0, dfr1, phr0, phr1, 0.0f, res, s_cc, s_i);
 	if (threadIdx.x == 0) {
 		S->ext_phase = res[1];
 		S->df_min = res[0];

@@ -23,7 +23,7 @@ DTYPE_F32, DTYPE_F16 = 0, 1
 EXPORTS = [
     "gctf_create", "gctf_destroy", "gctf_error_string", "gctf_set_stream", "gctf_synchronize",
     "gctf_rescale_setup", "gctf_rescale", "gctf_scaled_image", "gctf_setup1", "gctf_setup2",
-    "gctf_set_phase", "gctf_gen_half_spectrum", "gctf_do1d", "gctf_refine1d", "gctf_do2d",
+    "gctf_set_phase", "gctf_set_search1d", "gctf_gen_half_spectrum", "gctf_do1d", "gctf_refine1d", "gctf_do2d",
     "gctf_refine", "gctf_spect_size", "gctf_get_half_spect", "gctf_set_half_spect",
     "gctf_full_spectrum", "gctf_estimate_batch", "gctf_tune", "gctf_profile",
     "gctf_profile_report", "gctf_grid_search_batch",
@@ -118,6 +118,7 @@ def load():
         "gctf_setup1": (ci, [vp, cf, cf, cf, cf, cf, cf]),
         "gctf_setup2": (ci, [vp, ci, ci]),
         "gctf_set_phase": (ci, [vp, cf, cf]),
+        "gctf_set_search1d": (ci, [vp, cf, cf, cf]),
         "gctf_gen_half_spectrum": (ci, [vp, fp]),
         "gctf_do1d": (ci, [vp, ctypes.POINTER(GctfResult)]),
         "gctf_refine1d": (ci, [vp, cf, cf, ctypes.POINTER(GctfResult)]),
@@ -224,6 +225,11 @@ class Context:
 
     def set_phase(self, p0, rng):
         self.check(self.L.gctf_set_phase(self.h, p0, rng))
+
+    def set_search1d(self, df_lo=0.0, df_hi=0.0, step=0.0):
+        """Initial Do1D defocus scan [df_lo, df_hi] A in steps of `step` A
+        (all 0: the reference's 3000 p^2 .. 30000 p^2 in 500 steps)."""
+        self.check(self.L.gctf_set_search1d(self.h, df_lo, df_hi, step))
 
     def gen_half_spectrum(self, dev_pad_img):
         self.check(self.L.gctf_gen_half_spectrum(self.h, ctypes.c_void_p(dev_pad_img)))

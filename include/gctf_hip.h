@@ -86,6 +86,14 @@ GCTF_API int gctf_setup1(gctf_ctx* ctx, float kv, float cs_mm, float amp_contras
 GCTF_API int gctf_setup2(gctf_ctx* ctx, int nx, int ny);
 /* CFindCtfBase::SetPhase(float fInitPhase, float fPhaseRange) (:549). */
 GCTF_API int gctf_set_phase(gctf_ctx* ctx, float p0_deg, float range_deg);
+
+/* Initial defocus scan of Do1D: CFindCtf1D::mFindDefocus scans
+ * 3000 p^2 .. 30000 p^2 A (p the working pixel size) in 500 steps
+ * (CFindCtf1D.cpp:79-80, CFindDefocus1D.cpp:76-80); here [df_lo_A, df_hi_A]
+ * in steps of step_A instead, with the reference's point-count rule
+ * (int)(range / step) / 2 * 2 + 1.  All three 0 restore the reference scan.
+ * Applies to the single-image and the batched plain searches. */
+GCTF_API int gctf_set_search1d(gctf_ctx* ctx, float df_lo_A, float df_hi_A, float step_A);
 /* CFindCtfBase::GenHalfSpectrum(float* gfPadImg) (:553): device image of
  * Setup2's size with row pitch (nx/2+1)*2 floats. */
 GCTF_API int gctf_gen_half_spectrum(gctf_ctx* ctx, const float* dev_pad_img);

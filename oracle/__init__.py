@@ -62,6 +62,7 @@ def lib():
         "orc_fc_create": (_c_void, []),
         "orc_fc_destroy": (None, [_c_void]),
         "orc_fc_setup": (None, [_c_void, _c_float, _c_float, _c_float, _c_float, _c_int, _c_int, _c_float]),
+        "orc_fc_set_search1d": (None, [_c_void, _c_float, _c_float, _c_float]),
         "orc_fc_trace_enable": (None, [_c_void, _c_int]),
         "orc_fc_trace_count": (_c_int, [_c_void]),
         "orc_fc_trace_get": (None, [_c_void, _f32p]),
@@ -277,6 +278,10 @@ class FindCtf2D:
     def set_phase(self, p0, rng):
         self._L.orc_fc_set_phase(self._h, p0, rng)
 
+    def set_search1d(self, lo=0.0, hi=0.0, step=0.0):
+        """Initial Do1D scan [lo, hi] A in steps of `step` A (0: reference scan)."""
+        self._L.orc_fc_set_search1d(self._h, lo, hi, step)
+
     def do1d(self):
         self._L.orc_fc_do1d(self._h)
         return self.results()
@@ -308,9 +313,10 @@ class FindCtf2D:
 
 
 def estimate(img, pix, kv=300.0, cs=2.7, ac=0.07, tile=512, log_spect=0, phase=(0.0, 0.0),
-             trace=False):
+             trace=False, scan1d=None):
     """Rescale + GenHalfSpectrum + Do2D for one micrograph (the reference's
-    full-search package path, CFindSeriesCtfs.cpp:108-141)."""
+    full-search package path, CFindSeriesCtfs.cpp:108-141).  scan1d:
+    (lo, hi, step) A of the initial 1D scan, None for the reference's."""
     img = np.ascontiguousarray(img, np.float32)
     ny, nx = img.shape
     nnx, nny, pitch, pixn, _ = rescale_sizes(nx, ny, pix)
@@ -320,6 +326,8 @@ def estimate(img, pix, kv=300.0, cs=2.7, ac=0.07, tile=512, log_spect=0, phase=(
         fc.trace(True)
     fc.gen_half_spectrum(pad, nnx, nny)
     fc.set_phase(phase[0], max(phase[1], 0.0))
+    if scan1d is not None:
+        fc.set_search1d(*scan1d)
     res = fc.do2d()
     return res, fc
 

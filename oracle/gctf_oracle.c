@@ -740,6 +740,8 @@ typedef struct {
 	double ring_ss;
 	/* 1D CC */
 	float lo1d, hi1d;
+	/* initial Do1D scan [lo, hi] A, step A (0: the reference scan) */
+	float scan[3];
 	/* candidate trace */
 	orc_trace* trace;
 	int ntrace, trace_cap, trace_on;
@@ -876,6 +878,13 @@ ORC_EXPORT void orc_fc_get_spects(orc_fc* fc, double* raw, double* spect,
 
 ORC_EXPORT int orc_fc_lowpassed(orc_fc* fc) { return fc->lowpassed; }
 
+/* The initial Do1D scan: [lo, hi] A in steps of step A; all 0 restore
+ * 3000 p^2 .. 30000 p^2 in 500 steps (CFindCtf1D.cpp:79-80). */
+ORC_EXPORT void orc_fc_set_search1d(orc_fc* fc, float lo, float hi, float step)
+{
+	fc->scan[0] = lo; fc->scan[1] = hi; fc->scan[2] = step;
+}
+
 ORC_EXPORT void orc_fc_set_phase(orc_fc* fc, float p0, float range)
 {
 	fc->ext_phase = p0;                           /* CFindCtfBase.cpp:78-82 */
@@ -895,13 +904,16 @@ static float fc_cc1d(orc_fc* fc, float df_a, float phase_deg)
 	return orc_cc1d(ctf, fc->radial, n, fc->lo1d, fc->hi1d, 1.0f);
 }
 
-static void fc_search1d(orc_fc* fc, const float dfr[2], const float phr[2],
+/* step > 0 replaces the reference step (range / 500, at least 10 A); the
+ * point count keeps the reference rule (CFindDefocus1D.cpp:76-80). */
+static void fc_search1d(orc_fc* fc, const float dfr[2], const float phr[2], float step,
 	int stage, float* best_df, float* best_ph, float* best_cc)
 {
 	int ndf = 501;
 	float dfrange = dfr[1] - dfr[0];
 	float dfstep = dfrange / (float)(ndf - 1);
 	if (dfstep < 10) dfstep = 10.0f;
+	if (step > 0.0f) dfstep = step;
 	ndf = (int)(dfrange / dfstep) / 2 * 2 + 1;
 	int nps = 37;
 	float psrange = phr[1] - phr[0];
@@ -928,12 +940,13 @@ static void fc_search1d(orc_fc* fc, const float dfr[2], const float phr[2],
 static void fc_find_defocus(orc_fc* fc)
 {
 	float p2 = fc->ctf.pix * fc->ctf.pix;
-	float dfr[2] = { 3000.0f * p2, 30000.0f * p2 };
+	float dfr[2] = { 3000.0f * p2, 30000.0f * p2 };   /* CFindCtf1D.cpp:79-80 */
+	if (fc->scan[0] > 0.0f) { dfr[0] = fc->scan[0]; dfr[1] = fc->scan[1]; }
 	float phr[2];
 	phr[0] = fmaxf(fc->ext_phase - fc->phase_range / 2, 0.0f);
 	phr[1] = fminf(phr[0] + fc->phase_range, 150.0f);
 	float df, ph, cc;
-	fc_search1d(fc, dfr, phr, ORC_ST_1D, &df, &ph, &cc);
+	fc_search1d(fc, dfr, phr, fc->scan[2], ORC_ST_1D, &df, &ph, &cc);
 	fc->ext_phase = ph; fc->df_min = df; fc->df_max = df; fc->score = cc;
 }
 
@@ -949,7 +962,7 @@ static void fc_refine_defocus(orc_fc* fc, float range)
 	phr[0] = fmaxf(fc->ext_phase - prange / 2, 0.0f);
 	phr[1] = fminf(phr[1] + prange, 150.0f);        /* quirk, CFindCtf1D.cpp:104 */
 	float df, ph, cc;
-	fc_search1d(fc, dfr, phr, ORC_ST_1D_REFINE, &df, &ph, &cc);
+	fc_search1d(fc, dfr, phr, 0.0f, ORC_ST_1D_REFINE, &df, &ph, &cc);
 	fc->ext_phase = ph; fc->df_min = df; fc->df_max = df; fc->score = cc;
 }
 

@@ -123,6 +123,27 @@ def test_search_1d_2d_matches_oracle_on_same_spectrum():
     assert rg["ctf_res"] == pytest.approx(ro["ctf_res"], rel=1e-6)
 
 
+def test_search1d_configured_scan_matches_oracle():
+    """Do1D on the 3000-40000 A / 100 A scan (371 points) lands where the
+    oracle's does, on the same spectrum."""
+    img = _img(1024, 1.2, seed=13, df1=16500.0, df2=15800.0, az_deg=20.0)
+    ctx, pad, (nnx, nny, pixn) = _gpu_spectra(img, 1.2)
+    sp_g = ctx.get_half_spect(raw=False)
+    fc = O.FindCtf2D(pix=pixn)
+    fc.set_half_spect(sp_g.astype(np.float64))
+    fc.set_phase(0.0, 0.0)
+    fc.set_search1d(3000.0, 40000.0, 100.0)
+    ro = fc.do1d()
+    ctx.set_phase(0.0, 0.0)
+    ctx.set_search1d(3000.0, 40000.0, 100.0)
+    rg = ctx.do1d()
+    assert rg["df_min"] == ro["df_min"]
+    assert abs(rg["score"] - ro["score"]) <= 1e-6
+    ctx.set_search1d()
+    fc.set_search1d()
+    assert ctx.do1d()["df_min"] == fc.do1d()["df_min"]
+
+
 @pytest.mark.parametrize("seed,df1,df2,az", [(31, 9000.0, 8600.0, 5.0), (32, 26000.0, 25000.0, -75.0),
                                               (33, 15000.0, 14900.0, 33.0), (35, 33000.0, 32000.0, 0.0)])
 def test_search_result_identical_to_oracle(seed, df1, df2, az):

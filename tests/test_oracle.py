@@ -375,6 +375,30 @@ def test_search_recovers_noiseless_parameters():
     assert cnt[O.ST_INIT] == 1 and cnt[O.ST_ASTIG] == 2601
 
 
+def test_search1d_configured_scan():
+    """The Do1D scan set by set_search1d: 3000-40000 A at 100 A is 371 points
+    (the reference's (int)(range / step) / 2 * 2 + 1 rule), df = lo + i step
+    in float; after the refine stage (10 A steps) it lands within one refine
+    step of the reference scan's result."""
+    fc = O.FindCtf2D(pix=1.2)
+    fc.trace(True)
+    fc.set_half_spect(_template_spectrum(1.2, 21000.0, 20000.0, 35.0))
+    fc.set_phase(0.0, 0.0)
+    fc.set_search1d(3000.0, 40000.0, 100.0)
+    r = fc.do1d()
+    d1 = fc.get_trace()
+    d1 = d1[d1[:, 0] == O.ST_1D]
+    assert len(d1) == 371
+    want = np.float32(3000.0) + np.arange(371, dtype=np.float32) * np.float32(100.0)
+    assert np.array_equal(d1[:, 1], want)
+    fc.set_search1d()                              # back to the reference scan
+    fc.trace(False); fc.trace(True)
+    r0 = fc.do1d()
+    d1 = fc.get_trace()
+    assert (d1[:, 0] == O.ST_1D).sum() == 501
+    assert abs(r["df_min"] - r0["df_min"]) <= 10.0
+
+
 def test_refine_respects_window_caps():
     fc = O.FindCtf2D(pix=1.2)
     fc.trace(True)
