@@ -409,9 +409,16 @@ def cpu_baseline(img, pix, gpu_res, scan1d=None):
     ref, _ = O.estimate(img, pix, scan1d=scan1d)
     dt = time.perf_counter() - t0
     par = {k: round(abs(gpu_res[k] - ref[k]), 4) for k in ("df_min", "df_max", "azimuth", "score")}
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
     return ({"value": round(1.0 / dt, 5), "unit": "micrographs/s", "cores": 1, "kind": "port",
              "sample": f"1 micrograph {img.shape[1]}x{img.shape[0]} at {pix} A, full plain path "
-                       f"(rescale, spectrum, 1D+2D search), FP64 oracle, {dt:.1f} s"}, par)
+                       f"(rescale, spectrum, 1D+2D search), FP64 oracle, {dt:.1f} s",
+             "host": {"cpu": model, "nproc": os.cpu_count()}}, par)
 
 
 if __name__ == "__main__":
