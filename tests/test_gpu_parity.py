@@ -211,6 +211,24 @@ def test_batch_equals_single():
             assert rb2[i][k] == rs[k], (i, k)
 
 
+def test_batch_uses_configured_scan():
+    """The batched plain path runs the 1D scan set with set_search1d: a batch
+    of one equals the single-micrograph Do2D with the same scan."""
+    dev = _dev()
+    n, pix = 1024, 1.0
+    img = _img(n, pix, seed=41, df1=9000.0, df2=8700.0, az_deg=10.0)
+    d = torch.from_numpy(img[None].copy()).to(dev)
+    ctx = _ctx()
+    ctx.set_search1d(3000.0, 40000.0, 100.0)
+    rb = ctx.estimate_batch(d.data_ptr(), 1, n, n, pix)[0]
+    ctx1, _, _ = _gpu_spectra(img, pix)
+    ctx1.set_phase(0.0, 0.0)
+    ctx1.set_search1d(3000.0, 40000.0, 100.0)
+    rs = ctx1.do2d()
+    for k in ("df_min", "df_max", "azimuth", "score", "ctf_res"):
+        assert rb[k] == rs[k], (k, rb[k], rs[k])
+
+
 def test_full_spectrum_matches_oracle():
     img = _img(1024, 1.2, seed=9, df1=16000.0, df2=15000.0, az_deg=70.0)
     ctx, pad, (nnx, nny, pixn) = _gpu_spectra(img, 1.2)
