@@ -179,6 +179,7 @@ struct gctf_ctx {
 	// batch buffers
 	int cap = 0;  // images the per-image buffers hold
 	DevBuf cbufA, cbufB, padc, bsums, rowsum, rt, tpart;
+	DevBuf jobs;   // tile work lists of gctf_estimate_list
 	DevBuf raw, hsum, spect, mom, q, radial, ring_spec, ring_ss, states, part2d, refine, shell_cc, fullsp,
 		moments1, plans, shell_bins;
 	int kstride = 0;  // ring pixels reserved per image in ring_spec
@@ -595,7 +596,7 @@ int ensure_image_buffers(gctf_ctx* c, int n, int T, int ring_n)
 // c->rowsum (launch_spectrum then reads them instead of the image).
 template <typename Tin>
 int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long long in_stride, int n,
-	float* out, long long out_stride, const TileGrid* tg)
+	float* out, long long out_stride, const TileGrid* tg, float* rowsum_out)
 {
 	hipStream_t s = c->stream;
 	const int ncx = rp.nnx / 2 + 1;
@@ -635,7 +636,7 @@ int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long lon
 		dim3 grid((rp.nny + 1) / 2, n);
 		Fft1dPlan p = rp.c2r->plan();
 		const TileGrid g = tg ? *tg : TileGrid{};
-		float* rs = tg ? c->rowsum.as<float>() : nullptr;
+		float* rs = tg ? (rowsum_out ? rowsum_out : c->rowsum.as<float>()) : nullptr;
 		PROF(c, "k_rescale_rows_c2r", if (m <= 2048) k_rescale_rows_c2r<256, 2048><<<grid, 256, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs);
 			else if (m <= 4096) k_rescale_rows_c2r<512, 4096><<<grid, 512, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs);
 			else if (m <= 8192) k_rescale_rows_c2r<1024, 8192><<<grid, 1024, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs);
@@ -655,11 +656,12 @@ int ensure_rescale_buffers(gctf_ctx* c, const RescalePlan& rp, int n)
 }
 
 int launch_rescale(gctf_ctx* c, const RescalePlan& rp, const void* in, int dtype, long long in_stride,
-	int n, float* out, long long out_stride, const TileGrid* tg = nullptr)
+	int n, float* out, long long out_stride, const TileGrid* tg = nullptr, float* rowsum_out = nullptr)
 {
 	if (dtype == GCTF_DTYPE_F16)
-		return launch_rescale_t<_Float16>(c, rp, (const _Float16*)in, in_stride, n, out, out_stride, tg);
-	return launch_rescale_t<float>(c, rp, (const float*)in, in_stride, n, out, out_stride, tg);
+		return launch_rescale_t<_Float16>(c, rp, (const _Float16*)in, in_stride, n, out, out_stride, tg,
+			rowsum_out);
+	return launch_rescale_t<float>(c, rp, (const float*)in, in_stride, n, out, out_stride, tg, rowsum_out);
 }
 
 int ensure_spectrum_buffers(gctf_ctx* c, const TileGrid& g, int T, int n)
@@ -1986,6 +1988,57 @@ int gctf_estimate_series(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, 
 }
 
 // ---- uneven batch ------------------------------------------------------------
+// ---- tile work lists (uneven micrographs) -----------------------------------
+// The tile passes of a list run as one launch each over a group of jobs:
+// consecutive micrographs read with the same pixel type whose row-pass
+// output fits kListRtBytes.  Buffer offsets restart at every group.
+constexpr size_t kListRtBytes = 8ull << 30;
+
+struct ListGroup {
+	int j0 = 0, nj = 0;      // jobs [j0, j0 + nj)
+	int f16 = 0;             // pixels read as float16
+	int items = 0;           // row-pass items
+	int max_nr = 0;          // largest block-row count among jobs that need row sums
+	int max_np = 0, max_split = 0;
+	size_t pad = 0, rowsum = 0, sums = 0, rt = 0, part = 0;   // elements
+};
+
+template <typename Tin, int T>
+int launch_tile_list_t(gctf_ctx* c, const TileTables& tt, const TileJob* d_jobs, const ListGroup& G)
+{
+	hipStream_t s = c->stream;
+	constexpr int CX = T / 2 + 1, ROWS = 16, RG = 2, CPW = 4;
+	const TileJob* jobs = d_jobs + G.j0;
+	if (G.max_nr > 0)
+		PROF(c, "k_row_sums", k_row_sums_list<Tin><<<dim3(G.max_nr, G.nj), 256, 0, s>>>(jobs, c->rowsum.as<float>()));
+	PROF(c, "k_block_rows", k_block_rows_list<<<dim3((G.max_np + 255) / 256, G.nj), 256, 0, s>>>(jobs,
+		c->rowsum.as<float>(), c->bsums.as<double>()));
+	PROF(c, "k_tile_rows", k_tile_rows_list<Tin, T, ROWS, RG><<<G.items, (ROWS / 2) * (T / 8), 0, s>>>(jobs, G.nj,
+		c->bsums.as<double>(), tt.mask.as<float>(), tt.tw.as<float2>(), c->rt.as<float2>()));
+	PROF(c, "k_tile_cols", k_tile_cols_list<T, CPW><<<dim3((CX + CPW - 1) / CPW, G.max_split, G.nj), CPW * (T / 8),
+		0, s>>>(jobs, c->rt.as<float2>(), tt.tw.as<float2>(), c->tpart.as<float>()));
+	PROF(c, "k_tile_reduce", k_tile_reduce_list<<<dim3((T * CX + 255) / 256, G.nj), 256, 0, s>>>(jobs,
+		c->tpart.as<float>(), T * CX, c->raw.as<float>()));
+	HIPCHK(c, hipGetLastError());
+	return GCTF_OK;
+}
+
+template <int T>
+int launch_tile_list_tt(gctf_ctx* c, const TileTables& tt, const TileJob* d_jobs, const ListGroup& G)
+{
+	return G.f16 ? launch_tile_list_t<_Float16, T>(c, tt, d_jobs, G) : launch_tile_list_t<float, T>(c, tt, d_jobs, G);
+}
+
+int launch_tile_list(gctf_ctx* c, int T, const TileTables& tt, const TileJob* d_jobs, const ListGroup& G)
+{
+	switch (T) {
+	case 256: return launch_tile_list_tt<256>(c, tt, d_jobs, G);
+	case 512: return launch_tile_list_tt<512>(c, tt, d_jobs, G);
+	case 1024: return launch_tile_list_tt<1024>(c, tt, d_jobs, G);
+	default: return fail(c, GCTF_ERR_UNSUPPORTED, "tile size");
+	}
+}
+
 int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result* out)
 {
 	if (!ctx) return GCTF_ERR_ARG;
@@ -2019,33 +2072,92 @@ int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result
 	}
 	CHK(ensure_image_buffers(ctx, n, T, kmax));
 	ctx->kstride = kmax;
-	for (int i = 0; i < n; ++i) {
-		CHK(ensure_spectrum_buffers(ctx, it[i].g, T, 1));
-		if (it[i].rp.active) {
-			CHK(ensure_rescale_buffers(ctx, it[i].rp, 1));
-			CHK(ensure_buf(ctx, ctx->padc, sizeof(float) * (size_t)it[i].rp.pitch * it[i].rp.nny));
+	// tile work list: micrographs read as float32 (rescaled ones and float32
+	// inputs) first, then those read as float16; groups of consecutive jobs
+	const int CX = T / 2 + 1, NRB = T / 16;
+	std::vector<int> order;
+	for (int pass = 0; pass < 2; ++pass)
+		for (int i = 0; i < n; ++i) {
+			const bool f16 = !it[i].rp.active && imgs[i].dtype == GCTF_DTYPE_F16;
+			if ((int)f16 == pass) order.push_back(i);
 		}
+	std::vector<TileJob> jobs((size_t)n);
+	std::vector<ListGroup> groups;
+	std::vector<size_t> pad_off((size_t)n, 0);
+	for (int k = 0; k < n; ++k) {
+		const int i = order[k];
+		const TileGrid& g = it[i].g;
+		const int f16 = !it[i].rp.active && imgs[i].dtype == GCTF_DTYPE_F16;
+		const int ntiles = g.ntx * g.nty;
+		const size_t rt = (size_t)ntiles * CX * T;
+		if (groups.empty() || groups.back().f16 != f16
+			|| (groups.back().rt + rt) * sizeof(float2) > kListRtBytes) {
+			ListGroup G;
+			G.j0 = k; G.f16 = f16;
+			groups.push_back(G);
+		}
+		ListGroup& G = groups.back();
+		TileJob& J = jobs[(size_t)k];
+		J.img = nullptr;
+		J.pitch = it[i].rp.active ? it[i].rp.pitch : imgs[i].nx;
+		J.g = g;
+		J.rows_first = G.items;
+		J.nsplit = std::min(ctx->tile_splits, ntiles);
+		J.need_rows = it[i].rp.active ? 0 : 1;
+		J.raw_slot = i;
+		const int nr = (g.nby - 1) * g.step + g.bsize;
+		J.rowsum_off = (long long)G.rowsum; G.rowsum += (size_t)nr * g.nbx;
+		J.sums_off = (long long)G.sums; G.sums += (size_t)g.nbx * g.nby * kSumParts;
+		J.rt_off = (long long)G.rt; G.rt += rt;
+		J.part_off = (long long)G.part; G.part += (size_t)J.nsplit * T * CX;
+		if (it[i].rp.active) { pad_off[(size_t)i] = G.pad; G.pad += (size_t)it[i].rp.pitch * it[i].rp.nny; }
+		G.items += (ntiles * NRB + 1) / 2;
+		if (J.need_rows) G.max_nr = std::max(G.max_nr, nr);
+		G.max_np = std::max(G.max_np, g.nbx * g.nby * kSumParts);
+		G.max_split = std::max(G.max_split, J.nsplit);
+		G.nj += 1;
 	}
+	ListGroup most;
+	for (const ListGroup& G : groups) {
+		most.pad = std::max(most.pad, G.pad); most.rowsum = std::max(most.rowsum, G.rowsum);
+		most.sums = std::max(most.sums, G.sums); most.rt = std::max(most.rt, G.rt);
+		most.part = std::max(most.part, G.part);
+	}
+	for (int i = 0; i < n; ++i)
+		if (it[i].rp.active) CHK(ensure_rescale_buffers(ctx, it[i].rp, 1));
+	CHK(ensure_buf(ctx, ctx->padc, sizeof(float) * std::max<size_t>(most.pad, 1)));
+	CHK(ensure_buf(ctx, ctx->rowsum, sizeof(float) * most.rowsum));
+	CHK(ensure_buf(ctx, ctx->bsums, sizeof(double) * most.sums));
+	CHK(ensure_buf(ctx, ctx->rt, sizeof(float2) * most.rt));
+	CHK(ensure_buf(ctx, ctx->tpart, sizeof(float) * most.part));
+	CHK(ensure_buf(ctx, ctx->jobs, sizeof(TileJob) * (size_t)n));
+	for (int k = 0; k < n; ++k) {
+		const int i = order[k];
+		jobs[(size_t)k].img = it[i].rp.active ? (const void*)(ctx->padc.as<float>() + pad_off[(size_t)i])
+			: imgs[i].data;
+	}
+	// ordered on the context's stream (it may be a non-blocking one)
+	HIPCHK(ctx, hipMemcpyAsync(ctx->jobs.p, jobs.data(), sizeof(TileJob) * (size_t)n, hipMemcpyHostToDevice,
+		ctx->stream));
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
 	char tail[64];
 	snprintf(tail, sizeof tail, "/%d", ctx->tile_splits);
 	key += tail;
 	auto record = [&]() -> int {
 		k_state_init<<<(n + 63) / 64, 64, 0, ctx->stream>>>(ctx->states.as<SearchState>(), n,
 			ctx->prm.ext_phase_deg, std::max(ctx->prm.ext_phase_range_deg, 0.0f));
-		for (int i = 0; i < n; ++i) {
-			const gctf_image& im = imgs[i];
-			const RescalePlan& rp = it[i].rp;
-			const long long st = (long long)im.nx * im.ny;
-			if (rp.active) {
-				float* pad = ctx->padc.as<float>();
+		for (const ListGroup& G : groups) {
+			for (int k = G.j0; k < G.j0 + G.nj; ++k) {
+				const int i = order[k];
+				const gctf_image& im = imgs[i];
+				const RescalePlan& rp = it[i].rp;
+				if (!rp.active) continue;
+				const long long st = (long long)im.nx * im.ny;
 				const long long ps = (long long)rp.pitch * rp.nny;
-				CHK(launch_rescale(ctx, rp, im.data, im.dtype, st, 1, pad, ps, &it[i].g));
-				CHK(launch_spectrum<float>(ctx, it[i].g, *tt, pad, rp.pitch, ps, 1, i, true));
-			} else if (im.dtype == GCTF_DTYPE_F16) {
-				CHK(launch_spectrum<_Float16>(ctx, it[i].g, *tt, (const _Float16*)im.data, im.nx, st, 1, i));
-			} else {
-				CHK(launch_spectrum<float>(ctx, it[i].g, *tt, (const float*)im.data, im.nx, st, 1, i));
+				CHK(launch_rescale(ctx, rp, im.data, im.dtype, st, 1, ctx->padc.as<float>() + pad_off[(size_t)i], ps,
+					&it[i].g, ctx->rowsum.as<float>() + jobs[(size_t)k].rowsum_off));
 			}
+			CHK(launch_tile_list(ctx, T, *tt, ctx->jobs.as<TileJob>(), G));
 		}
 		CHK(launch_post(ctx, *tt, n));
 		for (int i = 0; i < n; ++i) CHK(fill_plans(ctx, i, 1, *it[i].sp));

@@ -32,6 +32,33 @@ struct TileGrid {
 	int bsize;           // block side (step, or tile when blocks == 0)
 };
 
+// One micrograph of a tile work list: the tile passes of an uneven list of
+// micrographs run as single launches over all their work items, each
+// workgroup finding its micrograph here.  Offsets are in elements of the
+// shared buffers.
+struct TileJob {
+	const void* img;          // pixels the tile passes read (the launch's Tin)
+	int pitch;                // row pitch in pixels
+	TileGrid g;
+	int rows_first;           // first row-pass item (prefix over the launch's jobs)
+	int nsplit;               // tile-column splits
+	int need_rows;            // 1: row partials still to be summed from img
+	int raw_slot;             // spectrum slot of the micrograph
+	long long rowsum_off, sums_off, rt_off, part_off;
+};
+
+// The job holding row-pass item `item`: the last with rows_first <= item.
+__device__ __forceinline__ int find_job(const TileJob* __restrict__ jobs, int njobs, int item)
+{
+	int lo = 0, hi = njobs - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (jobs[mid].rows_first <= item) lo = mid;
+		else hi = mid - 1;
+	}
+	return lo;
+}
+
 // Sum over a wavefront of the lanes' partial sums of a row segment (xor
 // butterfly, a fixed order); every lane returns the same value.
 __device__ __forceinline__ float wave_row_sum(float v)
@@ -207,16 +234,14 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r(co
 // from an image in memory, in the same order, so both give the same bits.
 constexpr int kSumParts = 8;
 
-// Grid (nr, images), 256 threads: one block row per workgroup, one block
-// column per wavefront.
+// One block row r per workgroup of 256 threads, one block column per
+// wavefront.  img and rowsum are the micrograph's own.
 template <typename Tin>
-__global__ __launch_bounds__(256) void k_row_sums(const Tin* __restrict__ img, long long img_stride,
-	int pitch, TileGrid g, float* __restrict__ rowsum)
+__device__ __forceinline__ void row_sums_body(const Tin* __restrict__ img, int pitch, const TileGrid& g,
+	float* __restrict__ rowsum, int r)
 {
-	const int r = blockIdx.x;
-	const int nr = (g.nby - 1) * g.step + g.bsize;
-	const Tin* src = img + blockIdx.y * img_stride + (long long)(g.offy + r) * pitch;
-	float* rs = rowsum + ((long long)blockIdx.y * nr + r) * g.nbx;
+	const Tin* src = img + (long long)(g.offy + r) * pitch;
+	float* rs = rowsum + (long long)r * g.nbx;
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 	for (int bi = wave; bi < g.nbx; bi += 4) {
 		const int x0 = g.offx + bi * g.step;
@@ -227,23 +252,56 @@ __global__ __launch_bounds__(256) void k_row_sums(const Tin* __restrict__ img, l
 	}
 }
 
+// Grid (nr, images).
+template <typename Tin>
+__global__ __launch_bounds__(256) void k_row_sums(const Tin* __restrict__ img, long long img_stride,
+	int pitch, TileGrid g, float* __restrict__ rowsum)
+{
+	const int nr = (g.nby - 1) * g.step + g.bsize;
+	row_sums_body<Tin>(img + blockIdx.y * img_stride, pitch, g, rowsum + (long long)blockIdx.y * nr * g.nbx,
+		blockIdx.x);
+}
+
+// Grid (largest nr, jobs): the jobs with need_rows.
+template <typename Tin>
+__global__ __launch_bounds__(256) void k_row_sums_list(const TileJob* __restrict__ jobs, float* __restrict__ rowsum)
+{
+	const TileJob& J = jobs[blockIdx.y];
+	const int nr = (J.g.nby - 1) * J.g.step + J.g.bsize;
+	if (!J.need_rows || (int)blockIdx.x >= nr) return;
+	row_sums_body<Tin>((const Tin*)J.img, J.pitch, J.g, rowsum + J.rowsum_off, blockIdx.x);
+}
+
 // Second step: part p of block (bi, bj) adds its rows' partials in order,
 // in FP64.  Grid (ceil(nbx nby kSumParts / 256), images).
-__global__ __launch_bounds__(256) void k_block_rows(const float* __restrict__ rowsum, TileGrid g,
-	double* __restrict__ sums)
+__device__ __forceinline__ void block_rows_body(const float* __restrict__ rs, const TileGrid& g,
+	double* __restrict__ sums, int i)
 {
 	const int nblk = g.nbx * g.nby;
-	const int i = blockIdx.x * 256 + threadIdx.x;
 	if (i >= nblk * kSumParts) return;
 	const int b = i / kSumParts, p = i % kSumParts;
 	const int bi = b % g.nbx, bj = b / g.nbx;
 	const int rows = (g.bsize + kSumParts - 1) / kSumParts;
 	const int r0 = bj * g.step + p * rows, r1 = bj * g.step + min(g.bsize, (p + 1) * rows);
-	const int nr = (g.nby - 1) * g.step + g.bsize;
-	const float* rs = rowsum + (long long)blockIdx.y * nr * g.nbx;
 	double acc = 0.0;
 	for (int r = r0; r < r1; ++r) acc += (double)rs[r * g.nbx + bi];
-	sums[((long long)blockIdx.y * nblk + b) * kSumParts + p] = acc;
+	sums[(long long)b * kSumParts + p] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_block_rows(const float* __restrict__ rowsum, TileGrid g,
+	double* __restrict__ sums)
+{
+	const int nr = (g.nby - 1) * g.step + g.bsize;
+	block_rows_body(rowsum + (long long)blockIdx.y * nr * g.nbx, g,
+		sums + (long long)blockIdx.y * g.nbx * g.nby * kSumParts, blockIdx.x * 256 + threadIdx.x);
+}
+
+// Grid (ceil(largest nbx nby kSumParts / 256), jobs).
+__global__ __launch_bounds__(256) void k_block_rows_list(const TileJob* __restrict__ jobs,
+	const float* __restrict__ rowsum, double* __restrict__ sums)
+{
+	const TileJob& J = jobs[blockIdx.y];
+	block_rows_body(rowsum + J.rowsum_off, J.g, sums + J.sums_off, blockIdx.x * 256 + threadIdx.x);
 }
 
 __device__ __forceinline__ double block_sum(const double* bs, int b)
@@ -263,11 +321,12 @@ __device__ __forceinline__ double block_sum(const double* bs, int b)
 // registers while the current one is transformed.  Grid
 // (ceil(ntiles * T/ROWS / G), images).  The transposing stage reuses the
 // transforms' LDS (the split halves wait in registers across a barrier).
+// The pass for item `item` (row blocks item*G ..) of one micrograph: im, bs
+// and rt are the micrograph's pixels, block sums and row-pass output.
 template <typename Tin, int T, int ROWS, int G>
-__global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_rows(const Tin* __restrict__ img,
-	long long img_stride, int pitch, TileGrid g, const double* __restrict__ sums,
-	const float* __restrict__ mask, const float2* __restrict__ tw,
-	float2* __restrict__ rt, long long rt_stride)
+__device__ __forceinline__ void tile_rows_body(const Tin* __restrict__ im, int pitch, const TileGrid& g,
+	const double* __restrict__ bs, const float* __restrict__ mask, const float2* __restrict__ tw,
+	float2* __restrict__ rt, int item)
 {
 	constexpr int FT = T / 8;              // threads per transform
 	constexpr int NG = ROWS / 2;           // transforms per workgroup
@@ -285,9 +344,6 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_ro
 	float2* stage = buf;
 	const int nblk = g.ntx * g.nty * NRB;
 	const int grp = threadIdx.x / FT, lane = threadIdx.x % FT;
-	const Tin* im = img + blockIdx.y * img_stride;
-	const long long nb = (long long)g.nbx * g.nby * kSumParts;
-	const double* bs = sums + blockIdx.y * nb;
 	float2* fb = buf + grp * FL;
 	// this thread's pixels of the two rows of row block b
 	float va[PX], vb[PX];
@@ -301,7 +357,7 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_ro
 			vb[i] = load_px(src, (long long)pitch + lane + i * FT);
 		}
 	};
-	const int b0 = blockIdx.x * G;
+	const int b0 = item * G;
 	if (b0 >= nblk) return;
 	load_rows(b0);
 	for (int it = 0; it < G; ++it) {
@@ -347,7 +403,7 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_ro
 			}
 		}
 		__syncthreads();
-		float2* dst = rt + blockIdx.y * rt_stride + (long long)t * CX * T + rb * ROWS;
+		float2* dst = rt + (long long)t * CX * T + rb * ROWS;
 		for (int i = threadIdx.x; i < CX * ROWS; i += NT) {
 			int k = i / ROWS, yy = i % ROWS;
 			dst[(long long)k * T + yy] = stage[k * SP + yy];
@@ -357,14 +413,37 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_ro
 	}
 }
 
+template <typename Tin, int T, int ROWS, int G>
+__global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_rows(const Tin* __restrict__ img,
+	long long img_stride, int pitch, TileGrid g, const double* __restrict__ sums,
+	const float* __restrict__ mask, const float2* __restrict__ tw,
+	float2* __restrict__ rt, long long rt_stride)
+{
+	tile_rows_body<Tin, T, ROWS, G>(img + blockIdx.y * img_stride, pitch, g,
+		sums + blockIdx.y * (long long)g.nbx * g.nby * kSumParts, mask, tw, rt + blockIdx.y * rt_stride, blockIdx.x);
+}
+
+// The same over a tile work list: grid (total items of the jobs).
+template <typename Tin, int T, int ROWS, int G>
+__global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_rows_list(
+	const TileJob* __restrict__ jobs, int njobs, const double* __restrict__ sums,
+	const float* __restrict__ mask, const float2* __restrict__ tw, float2* __restrict__ rt)
+{
+	const int j = find_job(jobs, njobs, blockIdx.x);
+	const TileJob& J = jobs[j];
+	tile_rows_body<Tin, T, ROWS, G>((const Tin*)J.img, J.pitch, J.g, sums + J.sums_off, mask, tw, rt + J.rt_off,
+		blockIdx.x - J.rows_first);
+}
+
 // Column pass: for CPW kx-columns and the tiles [t0, t1) of split s,
 // transform along y, take |F| and accumulate tile * (1/ntiles).  Writes the
 // DC-shifted partial average part[image][s][y][kx].  Grid
 // (ceil(CX/CPW), nsplit, images), CPW*(T/8) threads.
+// One micrograph: rt its row-pass output, part its split partials; column
+// group cg, split s.
 template <int T, int CPW>
-__global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols(const float2* __restrict__ rt,
-	long long rt_stride, int ntiles, int nsplit, float fac, const float2* __restrict__ tw,
-	float* __restrict__ part)
+__device__ __forceinline__ void tile_cols_body(const float2* __restrict__ src, int ntiles, int nsplit,
+	float fac, const float2* __restrict__ tw, float* __restrict__ part, int cg, int s)
 {
 	constexpr int FT = T / 8;
 	constexpr int CX = T / 2 + 1;
@@ -372,16 +451,14 @@ __global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols(const float2* __res
 	constexpr int FL = fft_lds_len(T);
 	__shared__ float2 buf[CPW * FL];
 	const int grp = threadIdx.x / FT, lane = threadIdx.x % FT;
-	const int kx = blockIdx.x * CPW + grp;
+	const int kx = cg * CPW + grp;
 	const bool live = kx < CX;
-	const int s = blockIdx.y;
 	const int per = (ntiles + nsplit - 1) / nsplit;
 	const int t0 = s * per, t1 = min(ntiles, t0 + per);
 	float acc[PER];
 #pragma unroll
 	for (int i = 0; i < PER; ++i) acc[i] = 0.f;
 	float2* fb = buf + grp * FL;
-	const float2* src = rt + blockIdx.z * rt_stride;
 	for (int t = t0; t < t1; ++t) {
 		if (live) {
 			const float2* col = src + ((long long)t * CX + kx) * T;
@@ -397,7 +474,7 @@ __global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols(const float2* __res
 		fft_sync<FT>();
 	}
 	if (!live) return;
-	float* dst = part + ((long long)blockIdx.z * nsplit + s) * (long long)T * CX;
+	float* dst = part + (long long)s * T * CX;
 #pragma unroll
 	for (int i = 0; i < PER; ++i) {
 		int y = lane + i * FT;
@@ -406,16 +483,52 @@ __global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols(const float2* __res
 	}
 }
 
+template <int T, int CPW>
+__global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols(const float2* __restrict__ rt,
+	long long rt_stride, int ntiles, int nsplit, float fac, const float2* __restrict__ tw,
+	float* __restrict__ part)
+{
+	constexpr int CX = T / 2 + 1;
+	tile_cols_body<T, CPW>(rt + blockIdx.z * rt_stride, ntiles, nsplit, fac, tw,
+		part + (long long)blockIdx.z * nsplit * T * CX, blockIdx.x, blockIdx.y);
+}
+
+// The same over a tile work list: grid (ceil(CX/CPW), largest nsplit, jobs).
+template <int T, int CPW>
+__global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols_list(const TileJob* __restrict__ jobs,
+	const float2* __restrict__ rt, const float2* __restrict__ tw, float* __restrict__ part)
+{
+	const TileJob& J = jobs[blockIdx.z];
+	if ((int)blockIdx.y >= J.nsplit) return;
+	const int ntiles = J.g.ntx * J.g.nty;
+	tile_cols_body<T, CPW>(rt + J.rt_off, ntiles, J.nsplit, 1.0f / (float)ntiles, tw, part + J.part_off,
+		blockIdx.x, blockIdx.y);
+}
+
 // Sum the split partials in split order: raw[image][i] = sum_s part[s][i].
+__device__ __forceinline__ void tile_reduce_body(const float* __restrict__ p, int nsplit, int n,
+	float* __restrict__ raw, int i)
+{
+	if (i >= n) return;
+	float acc = 0.f;
+	for (int s = 0; s < nsplit; ++s) acc += p[(long long)s * n + i];
+	raw[i] = acc;
+}
+
 __global__ __launch_bounds__(256) void k_tile_reduce(const float* __restrict__ part, int nsplit,
 	int n, float* __restrict__ raw)
 {
-	const int i = blockIdx.x * 256 + threadIdx.x;
-	if (i >= n) return;
-	const float* p = part + (long long)blockIdx.y * nsplit * n;
-	float acc = 0.f;
-	for (int s = 0; s < nsplit; ++s) acc += p[(long long)s * n + i];
-	raw[(long long)blockIdx.y * n + i] = acc;
+	tile_reduce_body(part + (long long)blockIdx.y * nsplit * n, nsplit, n, raw + (long long)blockIdx.y * n,
+		blockIdx.x * 256 + threadIdx.x);
+}
+
+// Grid (ceil(n / 256), jobs): raw slot per job.
+__global__ __launch_bounds__(256) void k_tile_reduce_list(const TileJob* __restrict__ jobs,
+	const float* __restrict__ part, int n, float* __restrict__ raw)
+{
+	const TileJob& J = jobs[blockIdx.y];
+	tile_reduce_body(part + J.part_off, J.nsplit, n, raw + (long long)J.raw_slot * n,
+		blockIdx.x * 256 + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
