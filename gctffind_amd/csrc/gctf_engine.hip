@@ -129,6 +129,10 @@ struct RescalePlan {
 	float pix_raw = 0.f, bin = 1.f, pixn = 1.f;
 	int nnx = 0, nny = 0, pitch = 0;
 	bool active = false;
+	// rows_half / c2r_half: the R2C / C2R row transforms run at half length
+	// (even/odd samples packed into one complex row); rows / c2r are then the
+	// half-length plans
+	bool rows_half = false, c2r_half = false;
 	FftDev *rows = nullptr, *cf = nullptr, *ci = nullptr, *c2r = nullptr;
 };
 
@@ -516,10 +520,15 @@ int make_rescale_plan(gctf_ctx* c, int nx, int ny, float pix, RescalePlan& rp)
 	rp.pitch = (rp.nnx / 2 + 1) * 2;
 	rp.active = bin > 1;
 	if (rp.active) {
-		CHK(get_fft(c, nx, &rp.rows));
+		// an even row length whose Bluestein working length exceeds one LDS
+		// transform runs as a half-length complex transform
+		auto half = [](int n) { return n % 2 == 0 && !is_pow2(n) && next_pow2(2 * n - 1) > 16384; };
+		rp.rows_half = half(nx);
+		rp.c2r_half = half(rp.nnx);
+		CHK(get_fft(c, rp.rows_half ? nx / 2 : nx, &rp.rows));
 		CHK(get_fft(c, ny, &rp.cf));
 		CHK(get_fft(c, rp.nny, &rp.ci));
-		CHK(get_fft(c, rp.nnx, &rp.c2r));
+		CHK(get_fft(c, rp.c2r_half ? rp.nnx / 2 : rp.nnx, &rp.c2r));
 	}
 	return GCTF_OK;
 }
@@ -610,6 +619,11 @@ int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long lon
 		const int m = rp.rows->m;
 		dim3 grid((rp.ny + 1) / 2, n);
 		Fft1dPlan p = rp.rows->plan();
+		if (rp.rows_half) {
+			if (m > 16384) return fail(c, GCTF_ERR_UNSUPPORTED, "row length");
+			PROF(c, "k_rescale_rows_r2c_half", k_rescale_rows_r2c_half<Tin, 1024, 16384><<<dim3(rp.ny, n), 1024, 0,
+				s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA));
+		} else
 		PROF(c, "k_rescale_rows_r2c", if (m <= 2048) k_rescale_rows_r2c<Tin, 256, 2048><<<grid, 256, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
 			else if (m <= 4096) k_rescale_rows_r2c<Tin, 512, 4096><<<grid, 512, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
 			else if (m <= 8192) k_rescale_rows_r2c<Tin, 1024, 8192><<<grid, 1024, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
@@ -637,6 +651,11 @@ int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long lon
 		Fft1dPlan p = rp.c2r->plan();
 		const TileGrid g = tg ? *tg : TileGrid{};
 		float* rs = tg ? (rowsum_out ? rowsum_out : c->rowsum.as<float>()) : nullptr;
+		if (rp.c2r_half) {
+			if (m > 16384) return fail(c, GCTF_ERR_UNSUPPORTED, "row length");
+			PROF(c, "k_rescale_rows_c2r_half", k_rescale_rows_c2r_half<1024, 16384><<<dim3(rp.nny, n), 1024, 0,
+				s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs));
+		} else
 		PROF(c, "k_rescale_rows_c2r", if (m <= 2048) k_rescale_rows_c2r<256, 2048><<<grid, 256, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs);
 			else if (m <= 4096) k_rescale_rows_c2r<512, 4096><<<grid, 512, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs);
 			else if (m <= 8192) k_rescale_rows_c2r<1024, 8192><<<grid, 1024, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs);

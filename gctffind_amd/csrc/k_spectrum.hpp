@@ -104,6 +104,36 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_r2c(co
 	}
 }
 
+// Rows too long for one LDS transform of length nx (nx even, not a power of
+// two, Bluestein length above MAXN): one row per workgroup as a complex
+// transform of length h = nx/2 of z[m] = x[2m] + i x[2m+1].  With
+// E = (Z[k] + conj Z[h-k]) / 2 and O = -i (Z[k] - conj Z[h-k]) / 2 the
+// transforms of the even and odd samples, X[k] = E[k] + exp(-2 pi i k/nx) O[k]
+// for k < ncx_out <= h.  plan is the length-h plan.  Grid (ny, images).
+template <typename Tin, int NT, int MAXN>
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_r2c_half(const Tin* __restrict__ img,
+	long long img_stride, int nx, int ny, Fft1dPlan plan, int ncx_out,
+	float2* __restrict__ out, long long out_stride)
+{
+	__shared__ float2 buf[fft_lds_len(MAXN)];
+	const int tid = threadIdx.x;
+	const int h = nx / 2;
+	const Tin* src = img + blockIdx.y * img_stride + (long long)blockIdx.x * nx;
+	for (int m = tid; m < h; m += NT) buf[pidx(m)] = make_float2(load_px(src, 2 * m), load_px(src, 2 * m + 1));
+	__syncthreads();
+	fft_any_lds<NT, MAXN, false>(buf, plan, tid);
+	float2* dst = out + blockIdx.y * out_stride + (long long)blockIdx.x * ncx_out;
+	for (int k = tid; k < ncx_out; k += NT) {
+		const float2 z = buf[pidx(k)];
+		const float2 zc = buf[pidx(k == 0 ? 0 : h - k)];
+		const float2 E = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
+		const float2 O = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+		float sn, cs;
+		sincospif((float)k / (float)h, &sn, &cs);      // angle pi k / h = 2 pi k / nx
+		dst[k] = make_float2(E.x + (cs * O.x + sn * O.y), E.y + (cs * O.y - sn * O.x));
+	}
+}
+
 // Complex transpose out[x][y] = in[y][x]; in is rows x cols.  32x32 tiles.
 __global__ __launch_bounds__(256) void k_transpose_c(const float2* __restrict__ in,
 	long long in_stride, int rows, int cols, float2* __restrict__ out, long long out_stride)
@@ -214,6 +244,63 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r(co
 			if (ra >= 0) rs[ra * g.nbx + bi] = sa;
 			if (two && ra + 1 < nr) rs[(ra + 1) * g.nbx + bi] = sb;
 		}
+	}
+}
+
+// C2R rows too long for one LDS transform of length nnx (as for the R2C rows
+// above): one row per workgroup.  With h = nnx/2 and X the Hermitian
+// extension of the half row, the inverse transform of length h of
+// F[k] = (X[k] + X[k+h]) + i exp(2 pi i k/nnx) (X[k] - X[k+h]) is
+// z[m] = x[2m] + i x[2m+1].  plan is the length-h plan.  Row partials as in
+// k_rescale_rows_c2r (same pixels, same order).  Grid (nny, images).
+template <int NT, int MAXN>
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r_half(const float2* __restrict__ in,
+	long long in_stride, int nny, int nnx, Fft1dPlan plan, float* __restrict__ out,
+	long long out_stride, int pitch, TileGrid g, float* __restrict__ rowsum)
+{
+	__shared__ float2 buf[fft_lds_len(MAXN)];
+	const int tid = threadIdx.x;
+	const int ncx = nnx / 2 + 1, h = nnx / 2;
+	const int y = blockIdx.x;
+	const float2* ga = in + blockIdx.y * in_stride + (long long)y * ncx;
+	auto fold = [&](int k) -> float2 {
+		float2 a = ga[k];                              // X[k], k < h
+		float2 b = ga[h - k];                          // X[k+h] = conj X[h-k]
+		b.y = -b.y;
+		if (k == 0) { a.y = 0.f; b.y = 0.f; }          // X[0], X[h] real
+		const float2 su = make_float2(a.x + b.x, a.y + b.y);
+		const float2 di = make_float2(a.x - b.x, a.y - b.y);
+		float sn, cs;
+		sincospif((float)k / (float)h, &sn, &cs);      // exp(i pi k / h)
+		const float2 t = make_float2(cs * di.x - sn * di.y, cs * di.y + sn * di.x);
+		return make_float2(su.x - t.y, su.y + t.x);    // su + i t
+	};
+	fft_any_lds_src<NT, MAXN, true>(buf, plan, tid, fold);
+	float* da = out + blockIdx.y * out_stride + (long long)y * pitch;
+	for (int x = tid; x < pitch; x += NT) {
+		float v = 0.f;
+		if (x < nnx) {
+			const float2 z = buf[pidx(x >> 1)];
+			v = (x & 1) ? z.y : z.x;
+		}
+		da[x] = v;
+	}
+	if (rowsum == nullptr) return;
+	const int nr = (g.nby - 1) * g.step + g.bsize;
+	const int ra = y - g.offy;
+	if (ra < 0 || ra >= nr) return;
+	float* rs = rowsum + (long long)blockIdx.y * nr * g.nbx;
+	const int wave = tid >> 6, lane = tid & 63;
+	for (int bi = wave; bi < g.nbx; bi += NT / 64) {
+		const int x0 = g.offx + bi * g.step;
+		float sa = 0.f;
+		for (int i = lane; i < g.bsize; i += 64) {
+			const int x = x0 + i;
+			const float2 z = buf[pidx(x >> 1)];
+			sa += (x & 1) ? z.y : z.x;
+		}
+		sa = wave_row_sum(sa);
+		if (lane == 0) rs[ra * g.nbx + bi] = sa;
 	}
 }
 

@@ -3,7 +3,9 @@ properties that do not depend on the size: a tilt series of 41 micrographs of
 3838x3710 at 1.35 A (the zero-tilt package equals the single-micrograph full
 search and the FP64 oracle; every package recovers its generating defocus),
 and an uneven list of 4096^2 float32 and 5760x4092 float16 micrographs at 0.8
-and 1.1 A (the list equals the single-micrograph batches)."""
+and 1.1 A (the list equals the single-micrograph batches), and an
+11520x8184 super-resolution micrograph at 0.5 A (equal to its 2x
+Fourier-binned copy)."""
 import math
 
 import numpy as np
@@ -76,3 +78,39 @@ def test_uneven_full_size_list():
             assert got[i][k] == one[k], (i, k, got[i][k], one[k])
         dm = 0.5 * (got[i]["df_min"] + got[i]["df_max"])
         assert abs(dm - spec[i][4]) < 500.0, (i, dm, spec[i][4])
+
+
+def test_super_resolution_11520x8184():
+    """An 11520x8184 micrograph at 0.5 A and its 2x Fourier-binned 5760x4092
+    copy at 1.0 A rescale to the same 4800x3410 Fourier crop at 1.2 A, so
+    their estimates agree; both recover the generating defocus, and in an
+    uneven float16 list each equals its single-micrograph batch."""
+    gctf = _setup()
+    from gctffind_amd import synth
+    ny, nx, pix = 8184, 11520, 0.5
+    dfm, ast, az = 21000.0, 600.0, 35.0
+    big = synth.micrograph(ny, nx, pix=pix, seed=250, df1=dfm + ast / 2, df2=dfm - ast / 2, az_deg=az,
+                           device="cuda")
+    f = torch.fft.rfft2(big.to(torch.float64))
+    h, w = ny // 2, nx // 2
+    crop = torch.cat([f[:h // 2, :w // 2 + 1], f[ny - h // 2:, :w // 2 + 1]], dim=0)
+    binned = (torch.fft.irfft2(crop, s=(h, w)) / 4.0).to(torch.float32).contiguous()
+    del f, crop
+    torch.cuda.synchronize()
+    ctx = gctf.Context()
+    rd = ctx.estimate_batch(big.data_ptr(), 1, nx, ny, pix)[0]
+    rb = ctx.estimate_batch(binned.data_ptr(), 1, w, h, 2 * pix)[0]
+    for r in (rd, rb):
+        assert abs(0.5 * (r["df_min"] + r["df_max"]) - dfm) < 500.0, r
+    assert abs(rd["df_min"] - rb["df_min"]) <= 20.0 and abs(rd["df_max"] - rb["df_max"]) <= 20.0, (rd, rb)
+    assert abs(rd["azimuth"] - rb["azimuth"]) <= 1.0, (rd, rb)
+    b16, s16 = big.to(torch.float16).contiguous(), binned.to(torch.float16).contiguous()
+    images = [dict(data=b16.data_ptr(), dtype=gctf.DTYPE_F16, nx=nx, ny=ny, pix=pix),
+              dict(data=s16.data_ptr(), dtype=gctf.DTYPE_F16, nx=w, ny=h, pix=2 * pix)]
+    torch.cuda.synchronize()
+    got = gctf.Context().estimate_list(images)
+    for i, im in enumerate(images):
+        one = gctf.Context().estimate_batch(im["data"], 1, im["nx"], im["ny"], im["pix"], dtype=im["dtype"])[0]
+        for k in ("df_min", "df_max", "azimuth", "score", "ctf_res"):
+            assert got[i][k] == one[k], (i, k, got[i][k], one[k])
+        assert abs(0.5 * (got[i]["df_min"] + got[i]["df_max"]) - dfm) < 500.0, (i, got[i])

@@ -42,9 +42,39 @@ def _img(n=1024, pix=1.2, seed=1, **kw):
 
 @pytest.mark.parametrize("shape,pix", [((1024, 1024), 1.0), ((1000, 900), 0.8), ((768, 1024), 1.2)])
 def test_rescale_matches_oracle(shape, pix):
-    dev = _dev()
     ny, nx = shape
-    img = _img(max(shape), pix)[:ny, :nx].copy()
+    _check_rescale(_img(max(shape), pix)[:ny, :nx].copy(), pix)
+
+
+@pytest.mark.parametrize("pix", [0.5, 1.0])
+def test_rescale_long_rows_matches_oracle(pix):
+    """Rows of 11520 pixels (the super-resolution width): the R2C row transform
+    runs at half length, even and odd samples packed into one complex row; at
+    1.0 A the C2R rows (9600 pixels) do as well."""
+    from gctffind_amd import synth
+    _check_rescale(synth.micrograph_np(256, 11520, pix=pix, seed=7), pix)
+
+
+def test_long_rows_batch_equals_single():
+    """Row partials of the half-length C2R rows equal k_row_sums on the
+    rescaled image: the batch (partials from the rescale) and the single path
+    (partials from memory) give the same bits."""
+    dev = _dev()
+    from gctffind_amd import synth
+    ny, nx, pix = 640, 11520, 1.0
+    img = synth.micrograph_np(ny, nx, pix=pix, seed=8, df1=16000.0, df2=15400.0, az_deg=40.0)
+    d = torch.from_numpy(img).to(dev)
+    rb = _ctx().estimate_batch(d.data_ptr(), 1, nx, ny, pix)[0]
+    ctx1, _, _ = _gpu_spectra(img, pix)
+    ctx1.set_phase(0.0, 0.0)
+    rs = ctx1.do2d()
+    for k in ("df_min", "df_max", "azimuth", "score", "ctf_res"):
+        assert rb[k] == rs[k], (k, rb[k], rs[k])
+
+
+def _check_rescale(img, pix):
+    dev = _dev()
+    ny, nx = img.shape
     ctx = _ctx()
     nnx, nny, pitch, pixn = ctx.rescale_setup(nx, ny, pix)
     onnx, onny, opitch, opixn, _ = O.rescale_sizes(nx, ny, pix)
