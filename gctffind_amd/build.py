@@ -51,7 +51,7 @@ def build_cli(verbose=True):
     cxx = shutil.which("g++") or "g++"
     srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES]
     tmp = CLI + ".tmp"
-    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-ffp-contract=off", f"-I{INCLUDE}",
+    cmd = [cxx, "-O2", "-std=c++17", "-pthread", "-Wall", "-ffp-contract=off", f"-I{INCLUDE}",
            f"-I{os.path.join(CSRC, 'host')}", "-o", tmp] + srcs + [
         f"-L{HERE}", "-lgctf_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib",
         "-L/opt/rocm/lib", "-lamdhip64"]

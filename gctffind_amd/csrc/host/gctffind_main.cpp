@@ -9,11 +9,15 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gctf_host.hpp"
@@ -210,6 +214,144 @@ void save_imod(const Options& o, const std::vector<Package>& p, bool tomo)
 	fclose(fp);
 }
 
+// CLoadImages::ThreadMain (CLoadImages.cpp:64-137): micrographs are read on a
+// thread of their own, in processing order, while the GPU works on earlier
+// ones.  At most `ahead` read micrographs wait in memory.
+class ImageLoader {
+public:
+	struct Item {
+		bool ok = false;
+		int nx = 0, ny = 0, mode = 0;
+		std::vector<float> px;
+	};
+	ImageLoader(const std::vector<Package>& pk, const std::vector<int>& order, size_t ahead)
+		: pk_(pk), order_(order), ahead_(ahead), items_(order.size()), ready_(order.size(), 0)
+	{
+		th_ = std::thread([this] { run(); });
+	}
+	~ImageLoader()
+	{
+		{
+			std::lock_guard<std::mutex> l(m_);
+			stop_ = true;
+		}
+		cv_.notify_all();
+		th_.join();
+	}
+	// The k-th micrograph of the processing order; waits until it is read.
+	Item take(size_t k)
+	{
+		std::unique_lock<std::mutex> l(m_);
+		cv_.wait(l, [&] { return ready_[k] != 0; });
+		Item it = std::move(items_[k]);
+		taken_ = k + 1;
+		l.unlock();
+		cv_.notify_all();
+		return it;
+	}
+
+private:
+	void run()
+	{
+		for (size_t k = 0; k < order_.size(); ++k) {
+			{
+				std::unique_lock<std::mutex> l(m_);
+				cv_.wait(l, [&] { return stop_ || k < taken_ + ahead_; });
+				if (stop_) return;
+			}
+			const Package& p = pk_[order_[k]];
+			Item it;
+			MrcReader r;
+			it.ok = r.Open(p.file) && r.ReadSection(p.section, it.px);
+			it.nx = r.Header().nx;
+			it.ny = r.Header().ny;
+			it.mode = r.Header().mode;
+			{
+				std::lock_guard<std::mutex> l(m_);
+				items_[k] = std::move(it);
+				ready_[k] = 1;
+			}
+			cv_.notify_all();
+		}
+	}
+	const std::vector<Package>& pk_;
+	const std::vector<int>& order_;
+	const size_t ahead_;
+	std::vector<Item> items_;
+	std::vector<char> ready_;
+	size_t taken_ = 0;
+	bool stop_ = false;
+	std::mutex m_;
+	std::condition_variable cv_;
+	std::thread th_;
+};
+
+// CAsyncSaveImages (CAsyncSaveImages.cpp:31-110): full spectra are written on
+// a thread of their own, in the order they are handed over.  Stack sections
+// go to one file opened at the first; serial inputs get one file each.
+class SpectrumWriter {
+public:
+	struct Job {
+		std::string file;
+		bool stack = false;
+		int section = 0, nz = 1, nx = 0, ny = 0;
+		std::vector<float> px;
+	};
+	SpectrumWriter() { th_ = std::thread([this] { run(); }); }
+	~SpectrumWriter() { finish(); }
+	void push(Job j)
+	{
+		{
+			std::lock_guard<std::mutex> l(m_);
+			q_.push_back(std::move(j));
+		}
+		cv_.notify_one();
+	}
+	// Writes what is queued, then closes the stack file.
+	void finish()
+	{
+		if (!th_.joinable()) return;
+		{
+			std::lock_guard<std::mutex> l(m_);
+			done_ = true;
+		}
+		cv_.notify_one();
+		th_.join();
+	}
+
+private:
+	void run()
+	{
+		std::unique_ptr<MrcWriter> stack;
+		for (;;) {
+			Job j;
+			{
+				std::unique_lock<std::mutex> l(m_);
+				cv_.wait(l, [&] { return done_ || !q_.empty(); });
+				if (q_.empty()) break;
+				j = std::move(q_.front());
+				q_.pop_front();
+			}
+			if (j.stack) {
+				if (!stack) {
+					stack = std::make_unique<MrcWriter>();
+					if (!stack->Open(j.file, j.nx, j.ny, j.nz, 1.0f)) fprintf(stderr, "cannot write %s\n", j.file.c_str());
+				}
+				stack->WriteSection(j.section, j.px.data());
+			} else {
+				MrcWriter w;
+				if (w.Open(j.file, j.nx, j.ny, 1, 1.0f)) { w.WriteSection(0, j.px.data()); w.Close(); }
+			}
+		}
+		if (stack) stack->Close();
+	}
+	std::deque<Job> q_;
+	bool done_ = false;
+	std::mutex m_;
+	std::condition_variable cv_;
+	std::thread th_;
+};
+
 // CAsyncSaveImages::mEmbedSerial (CAsyncSaveImages.cpp:112-132)
 std::string serial_out(const std::string& out, const std::string& serial)
 {
@@ -334,22 +476,24 @@ int main(int argc, char** argv)
 		for (int i = 0; i < n; ++i) order.push_back(i);
 	}
 
-	std::unique_ptr<MrcWriter> stack;
+	std::unique_ptr<SpectrumWriter> writer;
+	if (!o.out_mrc.empty()) writer = std::make_unique<SpectrumWriter>();
+	ImageLoader loader(pk, order, 2);
 	CRescaleImage rescale;
 	CFindCtf2D find;
 	int ref = -1;
 	int done = 0;
-	for (int ii : order) {
+	for (size_t k = 0; k < order.size(); ++k) {
+		const int ii = order[k];
 		Package& p = pk[ii];
-		MrcReader r;
-		std::vector<float> img;
-		if (!r.Open(p.file) || !r.ReadSection(p.section, img)) {
-			fprintf(stderr, "Error: cannot read %s (section %d, mode %d)\n", p.file.c_str(), p.section,
-				r.Header().mode);
+		ImageLoader::Item im = loader.take(k);
+		if (!im.ok) {
+			fprintf(stderr, "Error: cannot read %s (section %d, mode %d)\n", p.file.c_str(), p.section, im.mode);
 			return 1;
 		}
+		std::vector<float>& img = im.px;
 		auto t0 = std::chrono::steady_clock::now();
-		int size[2] = {r.Header().nx, r.Header().ny};
+		int size[2] = {im.nx, im.ny};
 		try {
 			// CFindSeriesCtfs::mRescaleImage + mSetupFindCtf (:108-131)
 			rescale.Setup(size, o.pix);
@@ -400,25 +544,28 @@ int main(int argc, char** argv)
 		       "   %4d  %8.2f  %8.2f  %6.2f %6.2f %9.5f\n\n",
 			base_name(p.file).c_str(), done + 1, n - 1 - done, p.idx + 1, p.df_min, p.df_max, p.azimuth,
 			p.ext_phase, p.score);
-		if (!o.out_mrc.empty()) {
+		if (writer) {
 			float* fs = find.GenFullSpectrum();
 			int ss[2];
 			find.GetSpectSize(ss, false);
 			if (fs) {
+				SpectrumWriter::Job j;
+				j.nx = ss[0];
+				j.ny = ss[1];
+				j.px.assign(fs, fs + (size_t)ss[0] * ss[1]);
+				delete[] fs;
 				if (tomo || o.serial == 0) {
-					if (!stack) {
-						stack = std::make_unique<MrcWriter>();
-						if (!stack->Open(o.out_mrc, ss[0], ss[1], n, 1.0f)) fprintf(stderr, "cannot write %s\n", o.out_mrc.c_str());
-					}
-					stack->WriteSection(p.idx, fs);
+					j.stack = true;
+					j.file = o.out_mrc;
+					j.section = p.idx;
+					j.nz = n;
 				} else {
 					std::string b = base_name(p.file), serial = b.substr(std::min(prefix.size(), b.size()));
 					size_t sf = suffix.empty() ? serial.find(".mrc") : serial.find(suffix);
 					if (sf != std::string::npos) serial = serial.substr(0, sf);
-					MrcWriter w;
-					if (w.Open(serial_out(o.out_mrc, serial), ss[0], ss[1], 1, 1.0f)) { w.WriteSection(0, fs); w.Close(); }
+					j.file = serial_out(o.out_mrc, serial);
 				}
-				delete[] fs;
+				writer->push(std::move(j));
 			}
 		}
 		double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -426,7 +573,7 @@ int main(int argc, char** argv)
 		done += 1;
 		if (done % 21 == 0) { save_ctf(o, pk); save_imod(o, pk, tomo); }
 	}
-	if (stack) stack->Close();
+	if (writer) writer->finish();
 	save_ctf(o, pk);
 	save_imod(o, pk, tomo);
 	gctf_destroy(SharedContext());

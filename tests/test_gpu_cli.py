@@ -23,10 +23,13 @@ def _gpu():
         pytest.skip("no GPU")
 
 
-def oracle_series(imgs, pix, tilts):
-    """The reference's tilt-series loop restated with the oracle."""
+def oracle_series(imgs, pix, tilts, tomo=True):
+    """The reference's package loop restated with the oracle: a tilt series
+    starts at zero tilt and keeps astigmatism and azimuth fixed in Refine;
+    other inputs go in file order and refine them within 0.04 and 30 degrees
+    (CFindSeriesCtfs.cpp:168-203)."""
     n = len(imgs)
-    z = int(np.argmin(np.abs(tilts)))
+    z = int(np.argmin(np.abs(tilts))) if tomo else 0
     order = [z] + [i for i in range(n) if i != z]
     res = [None] * n
     ref = None
@@ -47,7 +50,10 @@ def oracle_series(imgs, pix, tilts):
             dfr = [dm, max(dm * 0.5, 5000.0 * pix * pix)]
             mean = (rp["df_max"] + rp["df_min"]) * 0.5
             ast = [0.0 if mean <= 0 else (rp["df_max"] - rp["df_min"]) * 0.5 / mean, 0.0]
-            r = fc.refine(np.float32(dfr), np.float32(ast), [rp["azimuth"], 0.0], [rp["ext_phase"], 0.0])
+            if not tomo:
+                ast[1] = 0.04
+            r = fc.refine(np.float32(dfr), np.float32(ast), [rp["azimuth"], 0.0 if tomo else 30.0],
+                          [rp["ext_phase"], 0.0])
             if res[ref]["score"] < r["score"]:
                 ref = i
             if r["score"] <= 0:
@@ -90,3 +96,32 @@ def test_tilt_series_cli_matches_oracle(tmp_path):
     assert imod[0] == "1  0  0.0  0.0  0.0  3" and len(imod) == 6
     spec = read_mrc(out_mrc)
     assert spec.shape == (5, 512, 512) and np.isfinite(spec).all()
+
+
+def test_serial_folder_cli(tmp_path):
+    """-Serial 1: every micrograph of a folder is estimated (read ahead on the
+    loader thread, float32 and float16 files) and its full spectrum written to
+    a file of its own (on the writer thread)."""
+    _gpu()
+    from gctffind_amd import synth
+    pix = 1.35
+    truth = [15000.0, 16500.0, 18000.0]
+    for k, df in enumerate(truth):
+        img = synth.micrograph_np(1024, 1024, pix=pix, seed=320 + k, df1=df + 250.0, df2=df - 250.0,
+                                  az_deg=30.0 * k)
+        write_mrc(str(tmp_path / f"mic_{k + 1:03d}.mrc"), img, mode=12 if k == 1 else 2)
+    out_ctf, out_mrc = str(tmp_path / "ctf.txt"), str(tmp_path / "spec.mrc")
+    r = subprocess.run([CLI, "-InMrc", str(tmp_path / "mic_"), "-InSuffix", ".mrc", "-Serial", "1",
+                        "-OutMrc", out_mrc, "-OutCtf", out_ctf, "-PixSize", str(pix)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rows = [ln.split() for ln in open(out_ctf).read().splitlines() if not ln.startswith("#")]
+    assert len(rows) == 3
+    imgs = [read_mrc(str(tmp_path / f"mic_{k + 1:03d}.mrc"))[0].astype(np.float32) for k in range(3)]
+    ref = oracle_series(imgs, pix, np.zeros(3, np.float32), tomo=False)
+    for i, row in enumerate(rows):
+        dfmin, dfmax, az = map(float, row[2:5])
+        assert abs(dfmin - ref[i]["df_min"]) <= 5.0 and abs(dfmax - ref[i]["df_max"]) <= 5.0, (i, row, ref[i])
+        assert abs(az - ref[i]["azimuth"]) <= 0.5, (i, row, ref[i])
+        spec = read_mrc(str(tmp_path / f"spec{i + 1:03d}.mrc"))
+        assert spec.shape == (1, 512, 512) and np.isfinite(spec).all()
