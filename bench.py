@@ -83,6 +83,9 @@ def parse():
     ap.add_argument("--scan1d", default="",
                     help="initial 1D defocus scan lo,hi,step in A for the GPU and the oracle "
                          "(default: the reference's 3000 p^2 .. 30000 p^2 in 500 steps)")
+    ap.add_argument("--uneven-leg", type=int, default=0, metavar="N",
+                    help="also time gctf_estimate_list on N uneven micrographs (half 4096^2 f32, a quarter "
+                         "5760x4092 f16, a quarter 11520x8184 f16, 0.5-1.1 A); reported under config.uneven_list")
     ap.add_argument("--no-plain-leg", action="store_true",
                     help="learned mode: do not time the plain path beside it (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -211,6 +214,8 @@ def main():
             out["config"]["plain_ms_per_step"] = round(pms, 3)
             out["config"]["learned_over_plain"] = round(value / pv, 3)
         out["config"]["proposal_weights"] = {k: meta.get(k) for k in ("spectra", "micrograph", "epochs", "steps")}
+    if a.uneven_leg:
+        out["config"]["uneven_list"] = uneven_list(torch, gctf, ctx, a.uneven_leg, a.steps, a.warmup)
     dfe = [abs(0.5 * (r["df_min"] + r["df_max"]) - t[0]) for r, t in zip(res, truth)]
     out["config"]["median_abs_ddf_vs_truth_A"] = round(float(sorted(dfe)[len(dfe) // 2]), 1)
 
@@ -239,6 +244,36 @@ def timed(torch, run, n, steps, warmup):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return n * steps / dt, dt * 1000.0 / steps, res
+
+
+def uneven_list(torch, gctf, ctx, n, steps, warmup):
+    """The uneven-batch workload through gctf_estimate_list: n micrographs,
+    half 4096^2 float32, a quarter 5760x4092 float16 and a quarter 11520x8184
+    float16 (super-resolution), pixel 0.5-1.1 A, defocus log-uniform
+    5000-50000 A, astigmatism 0-1500 A, azimuth 0-180 degrees (seeds 200..)."""
+    import numpy as np
+    from gctffind_amd import synth
+    rng = np.random.default_rng(200)
+    q = n // 4
+    shapes = ([(4096, 4096, torch.float32, gctf.DTYPE_F32)] * (n - 2 * q)
+              + [(5760, 4092, torch.float16, gctf.DTYPE_F16)] * q
+              + [(11520, 8184, torch.float16, gctf.DTYPE_F16)] * q)
+    bufs, images, truth = [], [], []
+    for i, (nx, ny, tdt, dt) in enumerate(shapes):
+        pix, df1, df2, az = synth.random_params(rng)
+        img = synth.micrograph(ny, nx, pix=pix, seed=200 + i, df1=df1, df2=df2, az_deg=az, device="cuda")
+        bufs.append(img.to(tdt).contiguous())
+        del img
+        images.append(dict(data=bufs[-1].data_ptr(), dtype=dt, nx=nx, ny=ny, pix=pix))
+        truth.append(0.5 * (df1 + df2))
+    torch.cuda.synchronize()
+    value, ms, res = timed(torch, lambda: ctx.estimate_list(images), n, steps, warmup)
+    dfe = sorted(abs(0.5 * (r["df_min"] + r["df_max"]) - t) for r, t in zip(res, truth))
+    mpix = sum(im["nx"] * im["ny"] for im in images) / 1e6
+    return {"micrographs": n, "mix": f"{n - 2 * q} x 4096^2 f32, {q} x 5760x4092 f16, {q} x 11520x8184 f16",
+            "pixel_A": "0.5-1.1 (uniform)", "value": round(value, 2), "unit": "micrographs/s",
+            "ms_per_list": round(ms, 3), "input_mpix_per_list": round(mpix, 1),
+            "median_abs_ddf_vs_truth_A": round(float(dfe[len(dfe) // 2]), 1)}
 
 
 def evaluate(torch, ctx, a, buf):
