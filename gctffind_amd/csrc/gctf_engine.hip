@@ -133,6 +133,9 @@ struct RescalePlan {
 	// (even/odd samples packed into one complex row); rows / c2r are then the
 	// half-length plans
 	bool rows_half = false, c2r_half = false;
+	// cols_split: the column pass's forward transform runs as the transforms
+	// of the even and odd samples; cf is then the half-length plan
+	bool cols_split = false;
 	FftDev *rows = nullptr, *cf = nullptr, *ci = nullptr, *c2r = nullptr;
 };
 
@@ -525,8 +528,12 @@ int make_rescale_plan(gctf_ctx* c, int nx, int ny, float pix, RescalePlan& rp)
 		auto half = [](int n) { return n % 2 == 0 && !is_pow2(n) && next_pow2(2 * n - 1) > 16384; };
 		rp.rows_half = half(nx);
 		rp.c2r_half = half(rp.nnx);
+		rp.cols_split = half(ny);
+		if (rp.cols_split && rp.nny > 8192)
+			return fail(c, GCTF_ERR_UNSUPPORTED, "rescaled height " + std::to_string(rp.nny)
+				+ " above 8192 for a split column transform");
 		CHK(get_fft(c, rp.rows_half ? nx / 2 : nx, &rp.rows));
-		CHK(get_fft(c, ny, &rp.cf));
+		CHK(get_fft(c, rp.cols_split ? ny / 2 : ny, &rp.cf));
 		CHK(get_fft(c, rp.nny, &rp.ci));
 		CHK(get_fft(c, rp.c2r_half ? rp.nnx / 2 : rp.nnx, &rp.c2r));
 	}
@@ -639,6 +646,11 @@ int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long lon
 		const int m = std::max(rp.cf->m, rp.ci->m);
 		dim3 grid(8 * ((ncx + 7) / 8), n);
 		Fft1dPlan pf = rp.cf->plan(), pi = rp.ci->plan();
+		if (rp.cols_split) {
+			if (m > 16384) return fail(c, GCTF_ERR_UNSUPPORTED, "column length");
+			PROF(c, "k_rescale_cols_split", k_rescale_cols_split<1024, 16384><<<grid, 1024, 0, s>>>(B, stB, rp.ny,
+				rp.nny, ncx, pf, pi, A, stA));
+		} else
 		PROF(c, "k_rescale_cols", if (m <= 2048) k_rescale_cols<256, 2048><<<grid, 256, 0, s>>>(B, stB, rp.ny, rp.nny, ncx, pf, pi, A, stA);
 			else if (m <= 4096) k_rescale_cols<512, 4096><<<grid, 512, 0, s>>>(B, stB, rp.ny, rp.nny, ncx, pf, pi, A, stA);
 			else if (m <= 8192) k_rescale_cols<1024, 8192><<<grid, 1024, 0, s>>>(B, stB, rp.ny, rp.nny, ncx, pf, pi, A, stA);

@@ -181,6 +181,66 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_cols(const 
 	for (int y = tid; y < nny; y += NT) dst[(long long)y * ncx] = buf[pidx(y)];
 }
 
+// Columns too long for one LDS transform (ny even, not a power of two, above
+// MAXN/2; portrait super-resolution images): the forward transform of length
+// ny is split into the transforms of the even and odd samples (length
+// h = ny/2), X[k] = E[k mod h] + exp(-2 pi i k/ny) O[k mod h].  Each thread
+// parks the E values of its kept rows y' in the output column (out[y'][kx])
+// until O is done, and reads them back itself.  pf is the length-h plan;
+// nny <= MAXN/2.  Grid (8 ceil(ncx / 8), images).
+template <int NT, int MAXN>
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_cols_split(const float2* __restrict__ in,
+	long long in_stride, int ny, int nny, int ncx, Fft1dPlan pf, Fft1dPlan pi,
+	float2* __restrict__ out, long long out_stride)
+{
+	constexpr int JN = (MAXN / 2 + NT - 1) / NT;     // kept rows per thread
+	__shared__ float2 buf[fft_lds_len(MAXN)];
+	const int tid = threadIdx.x;
+	const int kx = xcd_group(blockIdx.x, ncx);
+	if (kx < 0) return;
+	const int h = ny / 2;
+	const float2* src = in + blockIdx.y * in_stride + (long long)kx * ny;
+	float2* dst = out + blockIdx.y * out_stride + kx;
+	for (int m = tid; m < h; m += NT) buf[pidx(m)] = src[2 * m];
+	__syncthreads();
+	fft_any_lds<NT, MAXN, false>(buf, pf, tid);
+#pragma unroll
+	for (int j = 0; j < JN; ++j) {
+		const int y = tid + j * NT;
+		if (y < nny) {
+			int k = y > nny / 2 ? y - nny + ny : y;
+			if (k >= h) k -= h;
+			dst[(long long)y * ncx] = buf[pidx(k)];
+		}
+	}
+	__syncthreads();
+	for (int m = tid; m < h; m += NT) buf[pidx(m)] = src[2 * m + 1];
+	__syncthreads();
+	fft_any_lds<NT, MAXN, false>(buf, pf, tid);
+	float2 x[JN];
+#pragma unroll
+	for (int j = 0; j < JN; ++j) {
+		const int y = tid + j * NT;
+		if (y < nny) {
+			const int k = y > nny / 2 ? y - nny + ny : y;
+			const float2 e = dst[(long long)y * ncx];
+			const float2 o = buf[pidx(k >= h ? k - h : k)];
+			float sn, cs;
+			sincospif((float)k / (float)h, &sn, &cs);  // angle pi k / h = 2 pi k / ny
+			x[j] = make_float2(e.x + (cs * o.x + sn * o.y), e.y + (cs * o.y - sn * o.x));
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int j = 0; j < JN; ++j) {
+		const int y = tid + j * NT;
+		if (y < nny) buf[pidx(y)] = x[j];
+	}
+	__syncthreads();
+	fft_any_lds<NT, MAXN, true>(buf, pi, tid);
+	for (int y = tid; y < nny; y += NT) dst[(long long)y * ncx] = buf[pidx(y)];
+}
+
 // Half-spectrum rows -> real rows of length nnx (C2R; imaginary parts of the
 // DC and Nyquist bins are dropped, i.e. the real part of the inverse of the
 // Hermitian extension).  Two rows per transform.  Output rows have `pitch`

@@ -5,7 +5,7 @@ search and the FP64 oracle; every package recovers its generating defocus),
 and an uneven list of 4096^2 float32 and 5760x4092 float16 micrographs at 0.8
 and 1.1 A (the list equals the single-micrograph batches), and an
 11520x8184 super-resolution micrograph at 0.5 A (equal to its 2x
-Fourier-binned copy)."""
+Fourier-binned copy and, up to the mirrored azimuth, to its transpose)."""
 import math
 
 import numpy as np
@@ -104,6 +104,11 @@ def test_super_resolution_11520x8184():
         assert abs(0.5 * (r["df_min"] + r["df_max"]) - dfm) < 500.0, r
     assert abs(rd["df_min"] - rb["df_min"]) <= 20.0 and abs(rd["df_max"] - rb["df_max"]) <= 20.0, (rd, rb)
     assert abs(rd["azimuth"] - rb["azimuth"]) <= 1.0, (rd, rb)
+    # portrait: the transposed micrograph (8184 wide, 11520 high) runs the split
+    # column transform; defocus is unchanged and the azimuth mirrors to 90 - az
+    rt = ctx.estimate_batch(big.t().contiguous().data_ptr(), 1, ny, nx, pix)[0]
+    assert abs(rt["df_min"] - rd["df_min"]) <= 30.0 and abs(rt["df_max"] - rd["df_max"]) <= 30.0, (rt, rd)
+    assert abs((90.0 - rd["azimuth"] - rt["azimuth"] + 90.0) % 180.0 - 90.0) <= 2.0, (rt, rd)
     b16, s16 = big.to(torch.float16).contiguous(), binned.to(torch.float16).contiguous()
     images = [dict(data=b16.data_ptr(), dtype=gctf.DTYPE_F16, nx=nx, ny=ny, pix=pix),
               dict(data=s16.data_ptr(), dtype=gctf.DTYPE_F16, nx=w, ny=h, pix=2 * pix)]

@@ -46,13 +46,25 @@ def test_rescale_matches_oracle(shape, pix):
     _check_rescale(_img(max(shape), pix)[:ny, :nx].copy(), pix)
 
 
-@pytest.mark.parametrize("pix", [0.5, 1.0])
-def test_rescale_long_rows_matches_oracle(pix):
+@pytest.mark.parametrize("shape,pix", [((256, 11520), 0.5), ((256, 11520), 1.0), ((11520, 256), 0.5),
+                                       ((11520, 256), 0.7)])
+def test_rescale_long_rows_matches_oracle(shape, pix):
     """Rows of 11520 pixels (the super-resolution width): the R2C row transform
     runs at half length, even and odd samples packed into one complex row; at
-    1.0 A the C2R rows (9600 pixels) do as well."""
+    1.0 A the C2R rows (9600 pixels) do as well.  Columns of 11520 pixels
+    (portrait): the column pass splits its forward transform into those of
+    the even and odd samples."""
     from gctffind_amd import synth
-    _check_rescale(synth.micrograph_np(256, 11520, pix=pix, seed=7), pix)
+    _check_rescale(synth.micrograph_np(shape[0], shape[1], pix=pix, seed=7), pix)
+
+
+def test_rescale_split_columns_limit():
+    """A split column transform whose rescaled height exceeds 8192 is refused
+    with an error, not computed."""
+    _dev()
+    from gctffind_amd import gctf
+    with pytest.raises(RuntimeError, match="8192"):
+        _ctx().rescale_setup(256, 11520, 1.0)
 
 
 def test_long_rows_batch_equals_single():
