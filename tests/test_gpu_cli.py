@@ -3,6 +3,7 @@ the reference's package logic (zero tilt first with a full search, then
 Refine around the best-scoring image so far; CFindSeriesCtfs.cpp:133-203)."""
 import math
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -125,3 +126,25 @@ def test_serial_folder_cli(tmp_path):
         assert abs(az - ref[i]["azimuth"]) <= 0.5, (i, row, ref[i])
         spec = read_mrc(str(tmp_path / f"spec{i + 1:03d}.mrc"))
         assert spec.shape == (1, 512, 512) and np.isfinite(spec).all()
+
+
+def test_serial_folder_cli_truncated_file(tmp_path):
+    """A truncated micrograph in a serial folder ends the run with a read
+    error (exit status 1) after the micrographs before it; the loader and
+    writer threads are joined, so the program exits instead of hanging."""
+    _gpu()
+    from gctffind_amd import synth
+    pix = 1.35
+    for k in range(3):
+        img = synth.micrograph_np(1024, 1024, pix=pix, seed=330 + k, df1=15250.0, df2=14750.0, az_deg=20.0)
+        write_mrc(str(tmp_path / f"mic_{k + 1:03d}.mrc"), img)
+    bad = tmp_path / "mic_002.mrc"
+    bad.write_bytes(bad.read_bytes()[:1024 + 4 * 1024 * 100])     # header + 100 of 1024 rows
+    r = subprocess.run([CLI, "-InMrc", str(tmp_path / "mic_"), "-InSuffix", ".mrc", "-Serial", "1",
+                        "-OutMrc", str(tmp_path / "spec.mrc"), "-OutCtf", str(tmp_path / "ctf.txt"),
+                        "-PixSize", str(pix)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "cannot read" in r.stderr and "mic_002.mrc" in r.stderr
+    assert re.search(r"mic_001\.mrc:\s+1 prcoessed", r.stdout), r.stdout[-2000:]   # the first was processed
+    spec = read_mrc(str(tmp_path / "spec001.mrc"))   # and its spectrum written
+    assert spec.shape == (1, 512, 512) and np.isfinite(spec).all()
