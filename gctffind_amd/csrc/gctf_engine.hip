@@ -134,8 +134,10 @@ struct RescalePlan {
 	// half-length plans
 	bool rows_half = false, c2r_half = false;
 	// cols_split: the column pass's forward transform runs as the transforms
-	// of the even and odd samples; cf is then the half-length plan
+	// of the even and odd samples; cf is then the half-length plan and
+	// cols_maxn the kernel's transform length (8192 or 16384)
 	bool cols_split = false;
+	int cols_maxn = 16384;
 	FftDev *rows = nullptr, *cf = nullptr, *ci = nullptr, *c2r = nullptr;
 };
 
@@ -532,6 +534,13 @@ int make_rescale_plan(gctf_ctx* c, int nx, int ny, float pix, RescalePlan& rp)
 		if (rp.cols_split && rp.nny > 8192)
 			return fail(c, GCTF_ERR_UNSUPPORTED, "rescaled height " + std::to_string(rp.nny)
 				+ " above 8192 for a split column transform");
+		// a column whose Bluestein length is 16384 also splits when its
+		// rescaled height fits 8192-point transforms: 73.7 KB of LDS, two
+		// workgroups per CU instead of one
+		if (!rp.cols_split && ny % 2 == 0 && !is_pow2(ny) && next_pow2(2 * ny - 1) == 16384 && rp.nny <= 4096) {
+			rp.cols_split = true;
+			rp.cols_maxn = 8192;
+		}
 		CHK(get_fft(c, rp.rows_half ? nx / 2 : nx, &rp.rows));
 		CHK(get_fft(c, rp.cols_split ? ny / 2 : ny, &rp.cf));
 		CHK(get_fft(c, rp.nny, &rp.ci));
@@ -647,9 +656,13 @@ int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long lon
 		dim3 grid(8 * ((ncx + 7) / 8), n);
 		Fft1dPlan pf = rp.cf->plan(), pi = rp.ci->plan();
 		if (rp.cols_split) {
-			if (m > 16384) return fail(c, GCTF_ERR_UNSUPPORTED, "column length");
-			PROF(c, "k_rescale_cols_split", k_rescale_cols_split<1024, 16384><<<grid, 1024, 0, s>>>(B, stB, rp.ny,
-				rp.nny, ncx, pf, pi, A, stA));
+			if (m > rp.cols_maxn) return fail(c, GCTF_ERR_UNSUPPORTED, "column length");
+			if (rp.cols_maxn == 8192)
+				PROF(c, "k_rescale_cols_split", k_rescale_cols_split<1024, 8192><<<grid, 1024, 0, s>>>(B, stB, rp.ny,
+					rp.nny, ncx, pf, pi, A, stA));
+			else
+				PROF(c, "k_rescale_cols_split", k_rescale_cols_split<1024, 16384><<<grid, 1024, 0, s>>>(B, stB, rp.ny,
+					rp.nny, ncx, pf, pi, A, stA));
 		} else
 		PROF(c, "k_rescale_cols", if (m <= 2048) k_rescale_cols<256, 2048><<<grid, 256, 0, s>>>(B, stB, rp.ny, rp.nny, ncx, pf, pi, A, stA);
 			else if (m <= 4096) k_rescale_cols<512, 4096><<<grid, 512, 0, s>>>(B, stB, rp.ny, rp.nny, ncx, pf, pi, A, stA);

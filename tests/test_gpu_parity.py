@@ -47,13 +47,14 @@ def test_rescale_matches_oracle(shape, pix):
 
 
 @pytest.mark.parametrize("shape,pix", [((256, 11520), 0.5), ((256, 11520), 1.0), ((11520, 256), 0.5),
-                                       ((11520, 256), 0.7)])
+                                       ((11520, 256), 0.7), ((8184, 256), 0.5)])
 def test_rescale_long_rows_matches_oracle(shape, pix):
     """Rows of 11520 pixels (the super-resolution width): the R2C row transform
     runs at half length, even and odd samples packed into one complex row; at
     1.0 A the C2R rows (9600 pixels) do as well.  Columns of 11520 pixels
     (portrait): the column pass splits its forward transform into those of
-    the even and odd samples."""
+    the even and odd samples; columns of 8184 pixels (the super-resolution
+    height) split too, onto 8192-point transforms."""
     from gctffind_amd import synth
     _check_rescale(synth.micrograph_np(shape[0], shape[1], pix=pix, seed=7), pix)
 
