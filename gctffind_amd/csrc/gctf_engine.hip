@@ -131,8 +131,10 @@ struct RescalePlan {
 	bool active = false;
 	// rows_half / c2r_half: the R2C / C2R row transforms run at half length
 	// (even/odd samples packed into one complex row); rows / c2r are then the
-	// half-length plans
+	// half-length plans and rows_maxn / c2r_maxn the kernels' transform
+	// lengths (8192 or 16384)
 	bool rows_half = false, c2r_half = false;
+	int rows_maxn = 16384, c2r_maxn = 16384;
 	// cols_split: the column pass's forward transform runs as the transforms
 	// of the even and odd samples; cf is then the half-length plan and
 	// cols_maxn the kernel's transform length (8192 or 16384)
@@ -530,6 +532,11 @@ int make_rescale_plan(gctf_ctx* c, int nx, int ny, float pix, RescalePlan& rp)
 		auto half = [](int n) { return n % 2 == 0 && !is_pow2(n) && next_pow2(2 * n - 1) > 16384; };
 		rp.rows_half = half(nx);
 		rp.c2r_half = half(rp.nnx);
+		// a row whose Bluestein length is 16384 also runs at half length, on
+		// 8192-point transforms: 73.7 KB of LDS, two workgroups per CU
+		auto half8 = [](int n) { return n % 2 == 0 && !is_pow2(n) && next_pow2(2 * n - 1) == 16384; };
+		if (half8(nx)) { rp.rows_half = true; rp.rows_maxn = 8192; }
+		if (half8(rp.nnx)) { rp.c2r_half = true; rp.c2r_maxn = 8192; }
 		rp.cols_split = half(ny);
 		if (rp.cols_split && rp.nny > 8192)
 			return fail(c, GCTF_ERR_UNSUPPORTED, "rescaled height " + std::to_string(rp.nny)
@@ -636,9 +643,13 @@ int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long lon
 		dim3 grid((rp.ny + 1) / 2, n);
 		Fft1dPlan p = rp.rows->plan();
 		if (rp.rows_half) {
-			if (m > 16384) return fail(c, GCTF_ERR_UNSUPPORTED, "row length");
-			PROF(c, "k_rescale_rows_r2c_half", k_rescale_rows_r2c_half<Tin, 1024, 16384><<<dim3(rp.ny, n), 1024, 0,
-				s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA));
+			if (m > rp.rows_maxn) return fail(c, GCTF_ERR_UNSUPPORTED, "row length");
+			if (rp.rows_maxn == 8192)
+				PROF(c, "k_rescale_rows_r2c_half", k_rescale_rows_r2c_half<Tin, 1024, 8192><<<dim3(rp.ny, n), 1024,
+					0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA));
+			else
+				PROF(c, "k_rescale_rows_r2c_half", k_rescale_rows_r2c_half<Tin, 1024, 16384><<<dim3(rp.ny, n), 1024,
+					0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA));
 		} else
 		PROF(c, "k_rescale_rows_r2c", if (m <= 2048) k_rescale_rows_r2c<Tin, 256, 2048><<<grid, 256, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
 			else if (m <= 4096) k_rescale_rows_r2c<Tin, 512, 4096><<<grid, 512, 0, s>>>(in, in_stride, rp.nx, rp.ny, p, ncx, A, stA);
@@ -677,9 +688,13 @@ int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long lon
 		const TileGrid g = tg ? *tg : TileGrid{};
 		float* rs = tg ? (rowsum_out ? rowsum_out : c->rowsum.as<float>()) : nullptr;
 		if (rp.c2r_half) {
-			if (m > 16384) return fail(c, GCTF_ERR_UNSUPPORTED, "row length");
-			PROF(c, "k_rescale_rows_c2r_half", k_rescale_rows_c2r_half<1024, 16384><<<dim3(rp.nny, n), 1024, 0,
-				s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs));
+			if (m > rp.c2r_maxn) return fail(c, GCTF_ERR_UNSUPPORTED, "row length");
+			if (rp.c2r_maxn == 8192)
+				PROF(c, "k_rescale_rows_c2r_half", k_rescale_rows_c2r_half<1024, 8192><<<dim3(rp.nny, n), 1024, 0,
+					s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs));
+			else
+				PROF(c, "k_rescale_rows_c2r_half", k_rescale_rows_c2r_half<1024, 16384><<<dim3(rp.nny, n), 1024, 0,
+					s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs));
 		} else
 		PROF(c, "k_rescale_rows_c2r", if (m <= 2048) k_rescale_rows_c2r<256, 2048><<<grid, 256, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs);
 			else if (m <= 4096) k_rescale_rows_c2r<512, 4096><<<grid, 512, 0, s>>>(A, stA, rp.nny, rp.nnx, p, out, out_stride, rp.pitch, g, rs);

@@ -47,14 +47,16 @@ def test_rescale_matches_oracle(shape, pix):
 
 
 @pytest.mark.parametrize("shape,pix", [((256, 11520), 0.5), ((256, 11520), 1.0), ((11520, 256), 0.5),
-                                       ((11520, 256), 0.7), ((8184, 256), 0.5)])
+                                       ((11520, 256), 0.7), ((8184, 256), 0.5), ((256, 5760), 0.8),
+                                       ((256, 5760), 1.0)])
 def test_rescale_long_rows_matches_oracle(shape, pix):
     """Rows of 11520 pixels (the super-resolution width): the R2C row transform
     runs at half length, even and odd samples packed into one complex row; at
     1.0 A the C2R rows (9600 pixels) do as well.  Columns of 11520 pixels
     (portrait): the column pass splits its forward transform into those of
     the even and odd samples; columns of 8184 pixels (the super-resolution
-    height) split too, onto 8192-point transforms."""
+    height) split too, onto 8192-point transforms, and so do rows of 5760
+    (R2C) and 4800 (C2R at 1.0 A) pixels."""
     from gctffind_amd import synth
     _check_rescale(synth.micrograph_np(shape[0], shape[1], pix=pix, seed=7), pix)
 
@@ -68,13 +70,14 @@ def test_rescale_split_columns_limit():
         _ctx().rescale_setup(256, 11520, 1.0)
 
 
-def test_long_rows_batch_equals_single():
-    """Row partials of the half-length C2R rows equal k_row_sums on the
-    rescaled image: the batch (partials from the rescale) and the single path
-    (partials from memory) give the same bits."""
+@pytest.mark.parametrize("nx", [11520, 5760])
+def test_long_rows_batch_equals_single(nx):
+    """Row partials of the half-length C2R rows (9600 and 4800 pixels) equal
+    k_row_sums on the rescaled image: the batch (partials from the rescale)
+    and the single path (partials from memory) give the same bits."""
     dev = _dev()
     from gctffind_amd import synth
-    ny, nx, pix = 640, 11520, 1.0
+    ny, pix = 640, 1.0
     img = synth.micrograph_np(ny, nx, pix=pix, seed=8, df1=16000.0, df2=15400.0, az_deg=40.0)
     d = torch.from_numpy(img).to(dev)
     rb = _ctx().estimate_batch(d.data_ptr(), 1, nx, ny, pix)[0]
