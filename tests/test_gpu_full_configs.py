@@ -5,7 +5,8 @@ search and the FP64 oracle; every package recovers its generating defocus),
 and an uneven list of 4096^2 float32 and 5760x4092 float16 micrographs at 0.8
 and 1.1 A (the list equals the single-micrograph batches), and an
 11520x8184 super-resolution micrograph at 0.5 A (equal to its 2x
-Fourier-binned copy and, up to the mirrored azimuth, to its transpose)."""
+Fourier-binned copy and, up to the mirrored azimuth, to its transpose), and
+a list of eight of them at 1.1 A that splits into two tile work groups."""
 import math
 
 import numpy as np
@@ -119,3 +120,28 @@ def test_super_resolution_11520x8184():
         for k in ("df_min", "df_max", "azimuth", "score", "ctf_res"):
             assert got[i][k] == one[k], (i, k, got[i][k], one[k])
         assert abs(0.5 * (got[i]["df_min"] + got[i]["df_max"]) - dfm) < 500.0, (i, got[i])
+
+
+def test_list_split_into_groups_equals_single_batches():
+    """Eight 11520x8184 float16 micrographs at 1.1 A need 1.18 GB of row-pass
+    output each, so the tile work list splits into two groups under its
+    8 GiB budget; every micrograph still equals its single-micrograph batch."""
+    gctf = _setup()
+    from gctffind_amd import synth
+    ny, nx, pix = 8184, 11520, 1.1
+    bufs, images, truth = [], [], []
+    for i in range(8):
+        dfm = 12000.0 + 3000.0 * i
+        img = synth.micrograph(ny, nx, pix=pix, seed=280 + i, df1=dfm + 300.0, df2=dfm - 300.0, az_deg=20.0 * i,
+                               device="cuda")
+        bufs.append(img.to(torch.float16).contiguous())
+        del img
+        images.append(dict(data=bufs[-1].data_ptr(), dtype=gctf.DTYPE_F16, nx=nx, ny=ny, pix=pix))
+        truth.append(dfm)
+    torch.cuda.synchronize()
+    got = gctf.Context().estimate_list(images)
+    for i, im in enumerate(images):
+        one = gctf.Context().estimate_batch(im["data"], 1, nx, ny, pix, dtype=gctf.DTYPE_F16)[0]
+        for k in ("df_min", "df_max", "azimuth", "score", "ctf_res"):
+            assert got[i][k] == one[k], (i, k, got[i][k], one[k])
+        assert abs(0.5 * (got[i]["df_min"] + got[i]["df_max"]) - truth[i]) < 500.0, (i, got[i])
