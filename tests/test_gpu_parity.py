@@ -47,8 +47,8 @@ def test_rescale_matches_oracle(shape, pix):
 
 
 @pytest.mark.parametrize("shape,pix", [((256, 11520), 0.5), ((256, 11520), 1.0), ((11520, 256), 0.5),
-                                       ((11520, 256), 0.7), ((8184, 256), 0.5), ((256, 5760), 0.8),
-                                       ((256, 5760), 1.0)])
+                                       ((11520, 256), 0.7), ((8184, 256), 0.5), ((8184, 256), 1.0),
+                                       ((256, 5760), 0.8), ((256, 5760), 1.0)])
 def test_rescale_long_rows_matches_oracle(shape, pix):
     """Rows of 11520 pixels (the super-resolution width): the R2C row transform
     runs at half length, even and odd samples packed into one complex row; at
