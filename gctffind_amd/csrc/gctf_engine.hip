@@ -850,7 +850,7 @@ int launch_1d(gctf_ctx* c, const TileTables& tt, int i0, int m, int mode, float 
 	const float* spect = c->spect.as<float>() + (size_t)i0 * T * cx;
 	float* radial = c->radial.as<float>() + (size_t)i0 * cx;
 	PROF(c, "k_radial", k_radial<<<dim3((cx + 63) / 64, m), 64, 0, s>>>(spect, cx, T, tt.roff.as<int>(),
-		tt.rcnt.as<int>(), tt.ridx.as<int>(), radial));
+		tt.rcnt.as<int>(), tt.ridx.as<int>(), radial, cx));
 	Search1DParams p{};
 	p.cmpx = cx;
 	p.mode = mode; p.init_df = init_df; p.range = range;
@@ -1503,6 +1503,46 @@ int gctf_full_spectrum(gctf_ctx* ctx, float* host_out)
 	return GCTF_OK;
 }
 
+int gctf_tile_grid(int nx, int ny, int tile, int* out6)
+{
+	if (!out6 || nx < 1 || ny < 1 || tile < 2) return GCTF_ERR_ARG;
+	const TileGrid g = make_grid(nx, ny, tile);
+	out6[0] = g.ntx; out6[1] = g.nty; out6[2] = g.offx; out6[3] = g.offy;
+	out6[4] = tile - g.step; out6[5] = g.step;
+	return GCTF_OK;
+}
+
+int gctf_ring_pixels(gctf_ctx* ctx, float pix, float res_lo, float res_hi, int* idx, int max, int* n)
+{
+	if (!ctx || !n || !(pix > 0)) return fail(ctx, GCTF_ERR_ARG, "null output or bad pixel size");
+	if ((res_lo > 0.f) != (res_hi > 0.f) || (res_lo > 0.f && !(res_lo > res_hi)))
+		return fail(ctx, GCTF_ERR_ARG, "resolution bounds: both 0, or res_lo > res_hi > 0");
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	SearchPlan* sp;
+	CHK(get_search_plan(ctx, ctx->prm.kv, ctx->prm.cs_mm, ctx->prm.amp_contrast, pix, ctx->prm.tile_size, &sp,
+		res_lo, res_hi));
+	*n = sp->ring.n;
+	if (idx && max > 0)
+		HIPCHK(ctx, hipMemcpy(idx, sp->idx.p, sizeof(int) * (size_t)std::min(max, sp->ring.n),
+			hipMemcpyDeviceToHost));
+	return GCTF_OK;
+}
+
+int gctf_radial_average(gctf_ctx* ctx, const float* d_spect, int n, float* d_radial, int ld)
+{
+	if (!ctx || !d_spect || !d_radial || n < 1) return fail(ctx, GCTF_ERR_ARG, "null spectra/output or n < 1");
+	const int T = ctx->prm.tile_size, cx = T / 2 + 1;
+	if (ld < cx) return fail(ctx, GCTF_ERR_ARG, "ld smaller than tile / 2 + 1");
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	TileTables* tt;
+	CHK(get_tile_tables(ctx, T, &tt));
+	k_radial<<<dim3((cx + 63) / 64, n), 64, 0, ctx->stream>>>(d_spect, cx, T, tt->roff.as<int>(),
+		tt->rcnt.as<int>(), tt->ridx.as<int>(), d_radial, ld);
+	HIPCHK(ctx, hipGetLastError());
+	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+	return GCTF_OK;
+}
+
 }  // extern "C"
 
 // ---- batched path ----------------------------------------------------------
@@ -1604,7 +1644,7 @@ int record_batch(gctf_ctx* c, const RescalePlan& rp, SearchPlan& sp, TileTables&
 		const int T = sp.T, cx = T / 2 + 1;
 		hipStream_t s = c->stream;
 		PROF(c, "k_radial", k_radial<<<dim3((cx + 63) / 64, n), 64, 0, s>>>(c->spect.as<float>(), cx, T, tt.roff.as<int>(),
-			tt.rcnt.as<int>(), tt.ridx.as<int>(), c->radial.as<float>()));
+			tt.rcnt.as<int>(), tt.ridx.as<int>(), c->radial.as<float>(), cx));
 		if (job.mode == 3) {
 			HIPCHK(c, hipMemcpy2DAsync(job.radial_out, sizeof(float) * job.radial_ld, c->radial.p,
 				sizeof(float) * cx, sizeof(float) * std::min(cx, job.radial_ld), n, hipMemcpyDeviceToDevice, s));
@@ -1636,14 +1676,25 @@ std::string shape_key(int nx, int ny, int dtype, float pix)
 	return k;
 }
 
-// Chunk and tile-split counts chosen by gctf_autotune for this shape.
-void apply_tuning(gctf_ctx* ctx, int nx, int ny, int dtype, float pix)
-{
-	auto it = ctx->tuned.find(shape_key(nx, ny, dtype, pix));
-	if (it == ctx->tuned.end()) return;
-	ctx->chunk = it->second.first;
-	ctx->tile_splits = it->second.second;
-}
+// Chunk and tile-split counts chosen by gctf_autotune for this shape, in
+// force for the lifetime of the scope only: the context's own values (the
+// library defaults or gctf_tune's) come back when the call returns, so one
+// shape's tuning never reaches a call on another shape.
+struct TuneScope {
+	gctf_ctx* c;
+	int chunk, splits;
+	TuneScope(gctf_ctx* ctx, int nx, int ny, int dtype, float pix)
+		: c(ctx), chunk(ctx->chunk), splits(ctx->tile_splits)
+	{
+		auto it = ctx->tuned.find(shape_key(nx, ny, dtype, pix));
+		if (it == ctx->tuned.end()) return;
+		ctx->chunk = it->second.first;
+		ctx->tile_splits = it->second.second;
+	}
+	~TuneScope() { c->chunk = chunk; c->tile_splits = splits; }
+	TuneScope(const TuneScope&) = delete;
+	TuneScope& operator=(const TuneScope&) = delete;
+};
 
 // Record the launches of `record` into an executable graph cached under
 // `key` (re-captured after any device buffer moved) and launch it; without
@@ -1679,7 +1730,7 @@ int run_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int
 	if (dtype != GCTF_DTYPE_F32 && dtype != GCTF_DTYPE_F16) return fail(ctx, GCTF_ERR_ARG, "dtype");
 	if (stride < (long long)nx * ny) return fail(ctx, GCTF_ERR_ARG, "stride smaller than one image");
 	HIPCHK(ctx, hipSetDevice(ctx->device));
-	apply_tuning(ctx, nx, ny, dtype, pix);
+	TuneScope tune(ctx, nx, ny, dtype, pix);
 	RescalePlan rp;
 	CHK(make_rescale_plan(ctx, nx, ny, pix, rp));
 	const int T = ctx->prm.tile_size;
@@ -1987,7 +2038,7 @@ int gctf_estimate_series(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, 
 	if (dtype != GCTF_DTYPE_F32 && dtype != GCTF_DTYPE_F16) return fail(ctx, GCTF_ERR_ARG, "dtype");
 	if (stride < (long long)nx * ny) return fail(ctx, GCTF_ERR_ARG, "stride smaller than one image");
 	HIPCHK(ctx, hipSetDevice(ctx->device));
-	apply_tuning(ctx, nx, ny, dtype, pix);
+	TuneScope tune(ctx, nx, ny, dtype, pix);
 	RescalePlan rp;
 	CHK(make_rescale_plan(ctx, nx, ny, pix, rp));
 	const int T = ctx->prm.tile_size;

@@ -262,7 +262,16 @@ private:
 			const Package& p = pk_[order_[k]];
 			Item it;
 			MrcReader r;
-			it.ok = r.Open(p.file) && r.ReadSection(p.section, it.px);
+			// a failed read (including a section buffer that cannot be
+			// allocated) is reported to the consumer as a read error; an
+			// exception must not end this thread, which would terminate
+			// the program
+			try {
+				it.ok = r.Open(p.file) && r.ReadSection(p.section, it.px);
+			} catch (const std::exception&) {
+				it.ok = false;
+				it.px.clear();
+			}
 			it.nx = r.Header().nx;
 			it.ny = r.Header().ny;
 			it.mode = r.Header().mode;

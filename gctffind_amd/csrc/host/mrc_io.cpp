@@ -65,7 +65,15 @@ bool MrcReader::Open(const std::string& path)
 	if (swap) { ni = (int16_t)bswap16((uint16_t)ni); nf = (int16_t)bswap16((uint16_t)nf); }
 	m_h.numintegers = ni; m_h.numfloats = nf;
 	m_h.swapped = swap;
-	if (MrcModeBytes(m_h.mode) == 0) { Close(); return false; }
+	if (MrcModeBytes(m_h.mode) == 0 || m_h.nsymbt < 0) { Close(); return false; }
+	// the header's sections must fit the file: a header that promises more
+	// data than the file holds is refused here, before any section buffer
+	// is sized from it
+	if (fseeko(m_fp, 0, SEEK_END) != 0) { Close(); return false; }
+	const long long fsize = (long long)ftello(m_fp);
+	const long long need = 1024LL + (long long)m_h.nsymbt
+		+ (long long)m_h.nz * (long long)m_h.nx * (long long)m_h.ny * MrcModeBytes(m_h.mode);
+	if (fsize < 0 || need > fsize) { Close(); return false; }
 	return true;
 }
 

@@ -765,8 +765,13 @@ __global__ __launch_bounds__(256) void k_gate(const double* __restrict__ mom, in
 		__syncthreads();
 	}
 	if (threadIdx.x == 0) {
-		double mean = r1[0] / n, m2 = r2[0] / n;
-		st[blockIdx.x].lowpassed = (m2 - mean * mean) < 1.0 ? 0 : 1;
+		// CFindCtfBase.cpp:171-174: the two moments come back as floats and
+		// fStd - fMean * fMean < 1 is evaluated in IEEE single (this library
+		// is built without contraction, so the product is rounded first)
+		const float fmean = (float)(r1[0] / n);
+		float fstd = (float)(r2[0] / n);
+		fstd = fstd - fmean * fmean;
+		st[blockIdx.x].lowpassed = fstd < 1.0f ? 0 : 1;
 	}
 }
 
@@ -858,7 +863,7 @@ __global__ __launch_bounds__(T / 8) void k_lp_y(const float* __restrict__ q,
 // in table order.
 __global__ __launch_bounds__(64) void k_radial(const float* __restrict__ spect, int cmpx, int cmpy,
 	const int* __restrict__ roff, const int* __restrict__ rcnt, const int* __restrict__ ridx,
-	float* __restrict__ radial)
+	float* __restrict__ radial, int ld)
 {
 	const float* in = spect + (long long)blockIdx.y * cmpx * cmpy;
 	const int r = blockIdx.x * 64 + threadIdx.x;
@@ -866,7 +871,7 @@ __global__ __launch_bounds__(64) void k_radial(const float* __restrict__ spect, 
 	double s = 0.0;
 	const int o = roff[r], n = rcnt[r];
 	for (int i = 0; i < n; ++i) s += (double)in[ridx[o + i]];
-	radial[(long long)blockIdx.y * cmpx + r] = (float)(s / n);
+	radial[(long long)blockIdx.y * ld + r] = (float)(s / n);
 }
 
 }  // namespace gctf

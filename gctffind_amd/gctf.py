@@ -25,7 +25,7 @@ EXPORTS = [
     "gctf_rescale_setup", "gctf_rescale", "gctf_scaled_image", "gctf_setup1", "gctf_setup2",
     "gctf_set_phase", "gctf_set_search1d", "gctf_gen_half_spectrum", "gctf_do1d", "gctf_refine1d", "gctf_do2d",
     "gctf_refine", "gctf_spect_size", "gctf_get_half_spect", "gctf_set_half_spect",
-    "gctf_full_spectrum", "gctf_estimate_batch", "gctf_tune", "gctf_profile",
+    "gctf_full_spectrum", "gctf_tile_grid", "gctf_ring_pixels", "gctf_radial_average", "gctf_estimate_batch", "gctf_tune", "gctf_profile",
     "gctf_profile_report", "gctf_grid_search_batch",
     "gctf_estimate_list", "gctf_estimate_series", "gctf_autotune",
     "gctf_learned_program_default", "gctf_radial_batch", "gctf_set_proposal_params",
@@ -128,6 +128,9 @@ def load():
         "gctf_get_half_spect": (ci, [vp, ci, fp]),
         "gctf_set_half_spect": (ci, [vp, fp]),
         "gctf_full_spectrum": (ci, [vp, fp]),
+        "gctf_tile_grid": (ci, [ci, ci, ci, fp]),
+        "gctf_ring_pixels": (ci, [vp, cf, cf, cf, fp, ci, fp]),
+        "gctf_radial_average": (ci, [vp, vp, ci, vp, ci]),
         "gctf_estimate_batch": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, fp]),
         "gctf_tune": (ci, [vp, ci, ci, ci]),
         "gctf_profile": (ci, [vp, ci]),
@@ -274,6 +277,23 @@ class Context:
         self.check(self.L.gctf_full_spectrum(self.h, _np_ptr(out)))
         return out
 
+    # ---- index tables ----------------------------------------------------
+    def ring_pixels(self, pix, res_lo=0.0, res_hi=0.0):
+        """Half-spectrum indices of the 2D correlation ring at pixel size pix."""
+        n = ctypes.c_int()
+        self.check(self.L.gctf_ring_pixels(self.h, pix, res_lo, res_hi, None, 0,
+                                           ctypes.cast(ctypes.pointer(n), ctypes.c_void_p)))
+        idx = np.zeros(n.value, np.int32)
+        self.check(self.L.gctf_ring_pixels(self.h, pix, res_lo, res_hi, _np_ptr(idx), idx.size,
+                                           ctypes.cast(ctypes.pointer(n), ctypes.c_void_p)))
+        return idx
+
+    def radial_average(self, dev_spect_ptr, n, dev_out_ptr, ld=None):
+        """GRadialAvg on n device half spectra into device rows of ld floats."""
+        ld = self.tile // 2 + 1 if ld is None else ld
+        self.check(self.L.gctf_radial_average(self.h, ctypes.c_void_p(dev_spect_ptr), n,
+                                              ctypes.c_void_p(dev_out_ptr), ld))
+
     # ---- batched path ----------------------------------------------------
     def estimate_batch(self, dev_ptr, n, nx, ny, pix, dtype=DTYPE_F32, stride=None):
         stride = nx * ny if stride is None else stride
@@ -367,6 +387,15 @@ class Context:
 
     def synchronize(self):
         self.check(self.L.gctf_synchronize(self.h))
+
+
+def tile_grid(nx, ny, tile=512):
+    """(tiles x, tiles y, offset x, offset y, overlap, step) of CGenAvgSpectrum::SetSizes."""
+    out = np.zeros(6, np.int32)
+    rc = load().gctf_tile_grid(nx, ny, tile, _np_ptr(out))
+    if rc != GCTF_OK:
+        raise GctfError(rc, "gctf_tile_grid")
+    return tuple(int(v) for v in out)
 
 
 def learned_program_default():

@@ -118,6 +118,26 @@ GCTF_API int gctf_set_half_spect(gctf_ctx* ctx, const float* host_spect);
  * |CTF| painted on the left half within the resolution ring. */
 GCTF_API int gctf_full_spectrum(gctf_ctx* ctx, float* host_out);
 
+/* ---- index tables ----------------------------------------------------
+ * The integer contracts the spectrum and search stages are built on, as
+ * the library computes them.  The reference forms them inside its kernels
+ * and setup code; these queries expose the library's own tables. */
+/* CGenAvgSpectrum::SetSizes (CGenAvgSpectrum.cpp:40-75) for an nx x ny
+ * image and tile size `tile`: out6 = tiles in x, tiles in y, x offset,
+ * y offset, overlap, step (tile i at offset + (i % ntx, i / ntx) * step). */
+GCTF_API int gctf_tile_grid(int nx, int ny, int tile, int* out6);
+/* Pixels y * (tile/2+1) + x of the half spectrum in the 2D correlation
+ * ring (GCC2D.cu:28-33, 152-156) at working pixel size `pix` A: the
+ * default ring [25 pix, max(2 pix / 0.8, 3.5)] A (CFindCtfBase.cpp:64-68)
+ * when res_lo = res_hi = 0, else [res_lo, res_hi] A.  *n gets the count;
+ * the first min(max, count) indices go to idx (host), in the order the
+ * search reads them. */
+GCTF_API int gctf_ring_pixels(gctf_ctx* ctx, float pix, float res_lo, float res_hi, int* idx, int max, int* n);
+/* GRadialAvg::DoIt (GRadialAvg.cu:11-53) on n device half spectra
+ * [n][tile][tile/2+1]: d_radial [n][ld] device floats, bins 0 .. tile/2,
+ * ld >= tile/2 + 1.  Returns after the averages are written. */
+GCTF_API int gctf_radial_average(gctf_ctx* ctx, const float* d_spect, int n, float* d_radial, int ld);
+
 /* ---- Batched path (no reference counterpart) --------------------------
  * Rescale + GenHalfSpectrum + Do2D (the reference's full-search package,
  * CFindSeriesCtfs.cpp:108-141) for n micrographs of one shape and pixel

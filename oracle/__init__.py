@@ -46,6 +46,7 @@ def lib():
         "orc_avg_spectrum": (None, [_f64p, _c_int, _c_int, _c_int, _c_int, _f64p]),
         "orc_rm_background": (None, [_f64p, _f64p, _c_int, _c_int, _c_int, _c_float]),
         "orc_moments": (None, [_f64p, _c_int, _f64p]),
+        "orc_gate": (_c_int, [_f64p, _c_int]),
         "orc_lowpass": (None, [_f64p, _c_int, _c_int]),
         "orc_radial_table": (_c_int, [_c_int, _c_int, _c_int, _c_void, _c_void]),
         "orc_radial_table_f64": (_c_int, [_c_int, _c_int, _c_void, _c_void, _c_void]),
@@ -158,6 +159,12 @@ def moments(spec):
     out = np.zeros(3, np.float64)
     lib().orc_moments(spec, spec.size, out)
     return out
+
+
+def gate(spec):
+    """1 when the flatness gate lets the low-pass run (CFindCtfBase.cpp:171-174)."""
+    spec = np.ascontiguousarray(spec, dtype=np.float64)
+    return int(lib().orc_gate(spec, spec.size))
 
 
 def lowpass(spec):

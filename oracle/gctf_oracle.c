@@ -414,6 +414,21 @@ ORC_EXPORT void orc_moments(const double* s, int n, double* out)
 	out[0] = m1; out[1] = m2; out[2] = m2 - m1 * m1;
 }
 
+/* The gate decision of CFindCtfBase.cpp:171-174: GCalcMoment2D returns the
+ * mean and E[x^2] as floats, and fStd = fStd - fMean * fMean is evaluated in
+ * IEEE single (compiled without contraction: the product is rounded before
+ * the subtraction).  The moments themselves are the FP64 sums rounded once.
+ * Returns 1 when the low-pass runs (fStd >= 1). */
+ORC_EXPORT int orc_gate(const double* s, int n)
+{
+	double m[3];
+	orc_moments(s, n, m);
+	const float fmean = (float)m[0];
+	float fstd = (float)m[1];
+	fstd = fstd - fmean * fmean;
+	return fstd < 1.0f ? 0 : 1;
+}
+
 /* ------------------------------------------------------------------------ */
 /* a13: spectrum low-pass.  CFindCtfBase.cpp:191-219; GCalcSpectrum.cu:47-69 */
 /* (unfold; row T read at yDst = 0 is taken mod T), GLowpass2D.cu:9-23,57-74 */
@@ -836,10 +851,8 @@ static void fc_background(orc_fc* fc)
 	size_t n = (size_t)fc->cmpx * fc->cmpy;
 	orc_rm_background(fc->raw, fc->spect, fc->log_spect, fc->cmpx, fc->cmpy,
 		1.0f / 15.0f);
-	double m[3];
-	orc_moments(fc->spect, (int)n, m);
 	fc->lowpassed = 0;
-	if (m[2] < 1.0) return;                      /* CFindCtfBase.cpp:174 */
+	if (!orc_gate(fc->spect, (int)n)) return;    /* CFindCtfBase.cpp:174 */
 	orc_lowpass(fc->spect, fc->cmpx, fc->cmpy);
 	fc->lowpassed = 1;
 }

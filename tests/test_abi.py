@@ -48,3 +48,21 @@ def test_create_without_gpu_fails_cleanly():
     h = ctypes.c_void_p()
     rc = L.gctf_create(None, 0, ctypes.byref(h))
     assert rc != 0 and not h.value
+
+
+def test_tile_grid_equals_oracle_and_shape_table():
+    """The library's tile grid (host integer code, no GPU work) is the
+    oracle's on every size of a sweep and the survey's shape table
+    (CGenAvgSpectrum.cpp:53-63)."""
+    if not os.path.exists(LIB):
+        pytest.skip("libgctf_hip.so not built")
+    import oracle as O
+    from gctffind_amd import gctf
+    table = {(3412, 3412): (12, 12, 42, 42), (3838, 3710): (13, 13, 127, 63), (4096, 4096): (15, 15, 0, 0),
+             (2400, 1704): (8, 5, 48, 84), (1024, 1024): (3, 3, 0, 0)}
+    for (nx, ny), want in table.items():
+        assert gctf.tile_grid(nx, ny, 512)[:4] == want
+    for tile in (256, 512, 1024):
+        for nx in range(tile, 12000, 173):
+            ny = tile + (nx * 7) % 9000
+            assert gctf.tile_grid(nx, ny, tile) == O.tile_grid(nx, ny, tile), (nx, ny, tile)

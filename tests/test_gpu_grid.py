@@ -103,3 +103,39 @@ def test_grid_full_size_recovers_truth():
                        n_ast=1, az_lo=float(az), az_hi=float(az) + 1.0, n_az=1, res_lo=30.0, res_hi=4.0)
             ref = O.grid_cc(sp.astype(np.float64), one, pixn)[0]
             assert abs(rs[i, j] - ref) <= 1e-5
+
+
+@pytest.mark.parametrize("n,cols", [(256, (0, 31, 32, 99, 128, 255)), (100, (0, 31, 32, 63, 64, 99)),
+                                    (40, (0, 31, 32, 39))])
+def test_grid_batch_sizes_match_oracle(n, cols):
+    """The GEMM's accumulator width follows the batch: n micrographs make
+    ceil(n / 32) column tiles per wave (k_grid_gemm<NT>).  n = 256 (the
+    256-micrograph workload) runs NT = 8, n = 100 NT = 4, n = 40 NT = 2.
+    Columns in different 32-column tiles are checked against the FP64
+    oracle on the same candidates, with the bounds of the small test."""
+    gctf = _setup()
+    from gctffind_amd import synth
+    pix = 1.2
+    grid = dict(df_lo=8000.0, df_hi=30000.0, n_df=45, ast_lo=0.0, ast_hi=0.06, n_ast=5, az_lo=-90.0,
+                az_hi=90.0, n_az=8, res_lo=30.0, res_hi=4.0)
+    rng = np.random.default_rng(n)
+    imgs = torch.empty((n, 1024, 1024), dtype=torch.float32, device="cuda")
+    for i in range(n):
+        dfm = float(rng.uniform(9000.0, 28000.0))
+        ast = float(rng.uniform(0.0, 900.0))
+        synth.micrograph(1024, 1024, pix=pix, seed=3000 + i, df1=dfm + ast / 2, df2=dfm - ast / 2,
+                         az_deg=float(rng.uniform(-90.0, 90.0)), device="cuda", out=imgs[i])
+    torch.cuda.synchronize()
+    ctx = gctf.Context()
+    res, cc, top, rs = ctx.grid_search_batch(imgs.data_ptr(), n, 1024, 1024, pix, grid, debug=True)
+    print(f"grid batch n={n} columns {cols}", flush=True)
+    for i in cols:
+        sp, pixn = _spectrum(gctf, imgs[i].cpu().numpy(), pix)
+        ref = O.grid_cc(sp.astype(np.float64), grid, pixn)
+        assert np.abs(cc[i] - ref).max() < 0.02, i
+        assert np.corrcoef(cc[i], ref)[0, 1] > 0.999, i
+        best = int(np.argmax(ref))
+        assert best in set(top[i].tolist()), (i, best, top[i])
+        for j, m in enumerate(top[i]):
+            assert abs(rs[i, j] - ref[m]) <= 1e-5, (i, j, m, rs[i, j], ref[m])
+        assert res[i]["score"] == pytest.approx(ref[best], abs=1e-5), i

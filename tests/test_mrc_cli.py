@@ -73,3 +73,43 @@ def test_unsupported_and_missing(tmp_path):
     assert "GCtfFind" in r.stdout
     r = run("-InMrc", "a.mrc", "-OutCtf", "A.MRC")     # same names (CCtfFindMain.cpp:50-62)
     assert r.returncode == 1 and "same name" in r.stderr
+
+
+def _patch_header(path, fmt, off, *vals):
+    import struct
+    b = bytearray(open(path, "rb").read())
+    struct.pack_into(fmt, b, off, *vals)
+    open(path, "wb").write(bytes(b))
+
+
+@pytest.mark.parametrize("case", ["huge", "negative_ext", "truncated", "ext_past_end"])
+def test_header_that_does_not_fit_the_file_is_refused(tmp_path, case):
+    """A header whose sections (1024 + extended header + nz * nx * ny * bytes)
+    do not fit in the file, or with a negative extended-header size, is
+    refused when the file is opened: no section buffer is sized from it, and
+    the command reports an error instead of allocating terabytes."""
+    f = str(tmp_path / "a.mrc")
+    d = np.random.default_rng(3).standard_normal((2, 20, 30)).astype(np.float32)
+    write_mrc(f, d, mode=2)
+    if case == "huge":
+        _patch_header(f, "<3i", 0, 1 << 19, 1 << 19, 2)      # 2 TiB of float32 promised
+    elif case == "negative_ext":
+        _patch_header(f, "<i", 92, -1024)
+    elif case == "truncated":
+        open(f, "wb").write(open(f, "rb").read()[:1024 + 4 * 30 * 20 + 100])
+    else:
+        _patch_header(f, "<i", 92, 1 << 20)
+    r = run("--mrcinfo", f)
+    assert r.returncode == 1 and r.stdout.strip() == "error", r.stdout + r.stderr
+
+
+def test_header_that_fits_exactly_is_read(tmp_path):
+    """The size check accepts a file that holds exactly the promised sections."""
+    f = str(tmp_path / "a.mrc")
+    d = np.arange(2 * 4 * 6, dtype=np.float32).reshape(2, 4, 6)
+    write_mrc(f, d, mode=2, tilts=[1.0, 2.0])
+    r = run("--mrcinfo", f)
+    assert r.returncode == 0, r.stdout + r.stderr
+    h, secs = parse(r.stdout)
+    assert (h["nx"], h["ny"], h["nz"]) == (6, 4, 2)
+    assert secs[1]["last"] == 47.0

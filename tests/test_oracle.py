@@ -205,6 +205,70 @@ def test_background_constant_input_is_zero():
     assert np.allclose(out, 0.0, atol=1e-12)
 
 
+# ---------------------------------------------------------------- a12 flatness gate
+def test_moments_match_numpy():
+    for seed, shape in ((10, (512, 257)), (11, (64, 33)), (12, (7, 5))):
+        s = rng(seed).standard_normal(shape) * 3.0 + 1.5
+        m = O.moments(s)
+        assert m[0] == pytest.approx(s.mean(), rel=1e-13, abs=1e-13)
+        assert m[1] == pytest.approx((s * s).mean(), rel=1e-13)
+        assert m[2] == pytest.approx(s.var(), rel=1e-10)
+
+
+def _two_level(mean, var, n=512 * 257):
+    """A spectrum of n values mean +- sqrt(var): its mean and variance are
+    exactly the given ones up to the FP64 rounding of the square."""
+    d = math.sqrt(var)
+    s = np.full(n, mean, np.float64)
+    s[0::2] += d
+    s[1::2] -= d
+    return s.reshape(512, 257)
+
+
+def _gate_f32(s):
+    # GCalcMoment2D's two float results and fStd - fMean * fMean in IEEE
+    # single, each operation rounded (numpy float32 arithmetic)
+    m1, m2 = np.float32(s.mean()), np.float32((s * s).mean())
+    return int(np.float32(m2 - np.float32(m1 * m1)) >= np.float32(1.0))
+
+
+def test_gate_is_the_reference_float_rule():
+    """The gate runs the low-pass iff fStd - fMean^2 >= 1 in IEEE single
+    (CFindCtfBase.cpp:171-174).  Near mean 4097 the float spacing of the
+    squares is 2, so
+    the single-precision variance differs from the FP64 one: spectra where
+    FP32 and FP64 disagree, in both directions, decide as numpy's float32
+    arithmetic does; ordinary spectra decide as their variance says.  Near
+    mean 6000 (spacing 4) a variance above 1 can round away entirely."""
+    disagree = {0: 0, 1: 0}
+    for mean in np.concatenate([np.arange(4090.0, 4100.0, 1.0), np.arange(6000.0, 6010.0, 1.0)]):
+        for var in (0.3, 0.7, 0.9, 1.1, 1.3, 1.7, 2.4):
+            s = _two_level(mean, var)
+            want = _gate_f32(s)
+            assert O.gate(s) == want, (mean, var)
+            if want != int(var >= 1.0):
+                disagree[want] += 1
+    assert disagree[0] > 0 and disagree[1] > 0, disagree
+    # small means: the float rule and the variance agree
+    for var, want in ((0.5, 0), (0.999, 0), (1.001, 1), (50.0, 1)):
+        assert O.gate(_two_level(3.0, var)) == want
+
+
+def test_flat_spectrum_skips_lowpass():
+    """A spectrum whose background-removed variance is below 1 is left
+    unfiltered (CFindCtfBase.cpp:174 returns before mLowpass)."""
+    raw = 1e-3 * (rng(13).random((512, 257)) + 1.0)
+    fc = O.FindCtf2D(pix=1.2)
+    fc.set_raw_spect(raw)
+    assert not fc.lowpassed()
+    _, sp, _ = fc.spects()
+    assert np.allclose(sp, O.rm_background(raw), rtol=0, atol=0)
+    fc.set_raw_spect(raw * 1e4)
+    assert fc.lowpassed()
+    _, sp, _ = fc.spects()
+    assert np.allclose(sp, O.lowpass(O.rm_background(raw * 1e4)), rtol=1e-12, atol=1e-9)
+
+
 # ---------------------------------------------------------------- a13 low-pass
 def test_lowpass_matches_numpy_transfer():
     t = 64
