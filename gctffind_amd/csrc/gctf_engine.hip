@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -158,6 +159,61 @@ struct GraphEntry {
 	int gen = -1;
 };
 
+// ---- static scratch of gctf_do2d_batch_static --------------------------------
+// Search buffers for at most kStaticMaxImages half spectra of the 512 tile,
+// in one file-scope device array: every GPU the library runs on holds its own
+// copy until the process ends, at its own address (kStaticScratchBytes, about
+// 18 MB).  Host code never passes `scratch` itself to a kernel (on the host
+// it names a stand-in, not device memory); it uses the address that
+// hipGetSymbolAddress returned for the device.
+constexpr int kStaticMaxImages = 16;
+constexpr int kStaticTile = 512;
+constexpr int kStaticCx = kStaticTile / 2 + 1;
+constexpr int kStaticMaxRing = 65536;   // ring pixels (zero-padded to 16) per spectrum
+constexpr int kStaticMaxBins = 368;     // integer radius bins of the 512 half spectrum (363 used)
+constexpr size_t st_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kStStates = 0;
+constexpr size_t kStPlans = kStStates + st_align(sizeof(SearchState) * kStaticMaxImages);
+constexpr size_t kStRefine = kStPlans + st_align(sizeof(ImgPlan) * kStaticMaxImages);
+constexpr size_t kStRingSs = kStRefine + st_align(sizeof(RefineArgs) * kStaticMaxImages);
+constexpr size_t kStRadial = kStRingSs + st_align(sizeof(double) * kStaticMaxImages);
+constexpr size_t kStShellCc = kStRadial + st_align(sizeof(float) * kStaticCx * kStaticMaxImages);
+constexpr size_t kStBins = kStShellCc + st_align(sizeof(float) * kStaticCx * kStaticMaxImages);
+constexpr size_t kStPart = kStBins + st_align(sizeof(double) * 6 * kStaticMaxBins * kStaticMaxImages);
+constexpr size_t kStRing = kStPart
+	+ st_align(sizeof(double2) * (size_t)kMaxCands * kPixChunks * kStaticMaxImages);
+constexpr size_t kStCtfMap = kStRing + st_align(sizeof(float) * (size_t)kStaticMaxRing * kStaticMaxImages);
+constexpr size_t kStaticScratchBytes = kStCtfMap
+	+ st_align(sizeof(float) * (size_t)kStaticTile * kStaticCx * kStaticMaxImages);
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+
+// Device address of `scratch` on each device, found once per device (with
+// that device current) and kept until the process ends.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// The address of this device's scratch; `device` must be current.
+hipError_t static_scratch_address(int device, void** out)
+{
+	*out = nullptr;
+	if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (!g_scratch_addr[device]) {
+		void* p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess) return e;
+		g_scratch_addr[device] = p;
+	}
+	*out = g_scratch_addr[device];
+	return hipSuccess;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -198,6 +254,10 @@ struct gctf_ctx {
 	// learned path: proposal-network parameters, activations, outputs
 	DevBuf net_params, net_out, props;
 	DevBuf series;  // tilt-series controller state
+	// device address of this GPU's static scratch (gctf_do2d_batch_static),
+	// looked up at creation; null with static_err set when the lookup failed
+	void* static_scratch = nullptr;
+	std::string static_err;
 	bool have_net = false;
 	NetBuffers* netb = nullptr;
 	std::vector<SearchState> h_states;
@@ -431,11 +491,24 @@ int get_tile_tables(gctf_ctx* c, int T, TileTables** out)
 // CCTFTheory::Setup (CCTFTheory.cpp:69-86) + the kernels' amplitude phase
 // (GCalcCTF2D.cu:88-97) + resolution range (CFindCtfBase.cpp:64-68) + CC
 // frequency bounds (CFindDefocus1D.cpp:128-135, CFindDefocus2D.cpp:86-92).
-int get_search_plan(gctf_ctx* c, float kv, float cs, float ac, float pix, int T, SearchPlan** out,
-	float res_lo = 0.f, float res_hi = 0.f)
+std::string plan_key(float kv, float cs, float ac, float pix, int T, float res_lo, float res_hi)
 {
 	char key[160];
 	snprintf(key, sizeof key, "%a/%a/%a/%a/%d/%a/%a", kv, cs, ac, pix, T, res_lo, res_hi);
+	return key;
+}
+
+// An existing plan with the default ring, or null; creates nothing.
+SearchPlan* find_search_plan(gctf_ctx* c, float kv, float cs, float ac, float pix, int T)
+{
+	auto it = c->splans.find(plan_key(kv, cs, ac, pix, T, 0.f, 0.f));
+	return it == c->splans.end() ? nullptr : it->second.get();
+}
+
+int get_search_plan(gctf_ctx* c, float kv, float cs, float ac, float pix, int T, SearchPlan** out,
+	float res_lo = 0.f, float res_hi = 0.f)
+{
+	const std::string key = plan_key(kv, cs, ac, pix, T, res_lo, res_hi);
 	auto it = c->splans.find(key);
 	if (it != c->splans.end()) { *out = it->second.get(); return GCTF_OK; }
 	auto sp = std::make_unique<SearchPlan>();
@@ -829,15 +902,39 @@ __global__ void k_fill_plans(ImgPlan* __restrict__ dst, int m, ImgPlan v)
 	if (i < m) dst[i] = v;
 }
 
-int fill_plans(gctf_ctx* c, int i0, int m, const SearchPlan& sp)
+// The plan of `sp` into dst[0 .. m).
+int fill_plans_at(gctf_ctx* c, ImgPlan* dst, int m, const SearchPlan& sp)
 {
-	if (sp.ring.n > c->kstride) return fail(c, GCTF_ERR_STATE, "ring larger than the reserved stride");
 	ImgPlan v;
 	v.cc = sp.cc;
 	v.ring = sp.ring;
 	v.lo1d = sp.lo1d;
 	v.hi1d = sp.hi1d;
-	k_fill_plans<<<(m + 63) / 64, 64, 0, c->stream>>>(c->plans.as<ImgPlan>() + i0, m, v);
+	k_fill_plans<<<(m + 63) / 64, 64, 0, c->stream>>>(dst, m, v);
+	HIPCHK(c, hipGetLastError());
+	return GCTF_OK;
+}
+
+int fill_plans(gctf_ctx* c, int i0, int m, const SearchPlan& sp)
+{
+	if (sp.ring.n > c->kstride) return fail(c, GCTF_ERR_STATE, "ring larger than the reserved stride");
+	return fill_plans_at(c, c->plans.as<ImgPlan>() + i0, m, sp);
+}
+
+// Radial average + CFindCtf1D::Do1D / Refine1D for m images: spectra
+// [m][T][T/2+1], radial averages [m][T/2+1], plans and states [m].
+int launch_1d_at(gctf_ctx* c, const TileTables& tt, const float* spect, float* radial, const ImgPlan* plans,
+	SearchState* st, int m, int mode, float init_df, float range)
+{
+	hipStream_t s = c->stream;
+	const int T = tt.T, cx = T / 2 + 1;
+	PROF(c, "k_radial", k_radial<<<dim3((cx + 63) / 64, m), 64, 0, s>>>(spect, cx, T, tt.roff.as<int>(),
+		tt.rcnt.as<int>(), tt.ridx.as<int>(), radial, cx));
+	Search1DParams p{};
+	p.cmpx = cx;
+	p.mode = mode; p.init_df = init_df; p.range = range;
+	p.scan_lo = c->scan_lo; p.scan_hi = c->scan_hi; p.scan_step = c->scan_step;
+	PROF(c, "k_search1d", k_search1d<<<m, 256, 0, s>>>(p, plans, radial, tt.w1d.as<double>(), st));
 	HIPCHK(c, hipGetLastError());
 	return GCTF_OK;
 }
@@ -845,20 +942,9 @@ int fill_plans(gctf_ctx* c, int i0, int m, const SearchPlan& sp)
 // Radial average + CFindCtf1D::Do1D / Refine1D for images [i0, i0 + m).
 int launch_1d(gctf_ctx* c, const TileTables& tt, int i0, int m, int mode, float init_df, float range)
 {
-	hipStream_t s = c->stream;
-	const int T = tt.T, cx = T / 2 + 1;
-	const float* spect = c->spect.as<float>() + (size_t)i0 * T * cx;
-	float* radial = c->radial.as<float>() + (size_t)i0 * cx;
-	PROF(c, "k_radial", k_radial<<<dim3((cx + 63) / 64, m), 64, 0, s>>>(spect, cx, T, tt.roff.as<int>(),
-		tt.rcnt.as<int>(), tt.ridx.as<int>(), radial, cx));
-	Search1DParams p{};
-	p.cmpx = cx;
-	p.mode = mode; p.init_df = init_df; p.range = range;
-	p.scan_lo = c->scan_lo; p.scan_hi = c->scan_hi; p.scan_step = c->scan_step;
-	PROF(c, "k_search1d", k_search1d<<<m, 256, 0, s>>>(p, c->plans.as<ImgPlan>() + i0, radial,
-		tt.w1d.as<double>(), c->states.as<SearchState>() + i0));
-	HIPCHK(c, hipGetLastError());
-	return GCTF_OK;
+	const int cx = tt.T / 2 + 1;
+	return launch_1d_at(c, tt, c->spect.as<float>() + (size_t)i0 * tt.T * cx, c->radial.as<float>() + (size_t)i0 * cx,
+		c->plans.as<ImgPlan>() + i0, c->states.as<SearchState>() + i0, m, mode, init_df, range);
 }
 
 // Device pointers of the 2D search for images [i0, i0 + m).
@@ -1032,6 +1118,22 @@ __global__ void k_state_init(SearchState* st, int n, float p0, float range)
 	st[i] = s;
 }
 
+// Search results in the ABI's layout, written on the device.
+__global__ void k_results_out(const SearchState* __restrict__ st, int n, gctf_result* __restrict__ out)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const SearchState& s = st[i];
+	gctf_result r;
+	r.df_min = s.df_min;
+	r.df_max = s.df_max;
+	r.azimuth_deg = s.ast_ang;
+	r.ext_phase_deg = s.ext_phase;
+	r.score = s.score;
+	r.ctf_res = s.ctf_res;
+	out[i] = r;
+}
+
 int read_results(gctf_ctx* c, int n, gctf_result* out)
 {
 	HIPCHK(c, hipMemcpyAsync(c->h_pinned, c->states.p, sizeof(SearchState) * n, hipMemcpyDeviceToHost,
@@ -1150,6 +1252,14 @@ int gctf_create(const gctf_params* params, int device, gctf_ctx** out)
 	c->prm = p;
 	if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) return GCTF_ERR_HIP;
 	c->stream = c->own_stream;
+	// this device's static scratch (gctf_do2d_batch_static); a failed lookup
+	// disables that entry point only
+	const hipError_t se = static_scratch_address(device, &c->static_scratch);
+	if (se != hipSuccess) {
+		c->static_scratch = nullptr;
+		c->static_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se);
+		(void)hipGetLastError();
+	}
 	*out = c.release();
 	return GCTF_OK;
 }
@@ -1411,6 +1521,67 @@ int gctf_do2d(gctf_ctx* ctx, gctf_result* out)
 	CHK(launch_1d(ctx, *tt, 0, 1, 0, 0.f, 0.f));
 	CHK(launch_2d(ctx, ctx->prm.tile_size, 0, 1, NX_BEGIN_DO2D, prange));
 	return single_result(ctx, out);
+}
+
+int gctf_do2d_batch_static(gctf_ctx* ctx, const float* d_spect, int n, gctf_result* d_out)
+{
+	if (!ctx) return GCTF_ERR_ARG;
+	if (!d_spect || !d_out || n < 1) return fail(ctx, GCTF_ERR_ARG, "null spectra/output or n < 1");
+	if (n > kStaticMaxImages)
+		return fail(ctx, GCTF_ERR_ARG, "gctf_do2d_batch_static takes at most " + std::to_string(kStaticMaxImages)
+			+ " spectra per call (the size of its static scratch), got " + std::to_string(n));
+	if (ctx->prm.tile_size != kStaticTile)
+		return fail(ctx, GCTF_ERR_UNSUPPORTED, "gctf_do2d_batch_static needs tile size 512");
+	CHK(need(ctx, ctx->setup1, "Setup1 has not been called"));
+	if (ctx->prof)
+		return fail(ctx, GCTF_ERR_STATE, "kernel profiling is on (it creates events); turn it off for "
+			"gctf_do2d_batch_static");
+	if (!ctx->static_scratch)
+		return fail(ctx, GCTF_ERR_HIP, ctx->static_err.empty() ? "no static scratch" : ctx->static_err);
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	// Setup1 made the plan of its parameters and the tile tables: looked up
+	// here, never created
+	SearchPlan* sp = find_search_plan(ctx, ctx->s_kv, ctx->s_cs, ctx->s_ac, ctx->s_pix, kStaticTile);
+	auto tit = ctx->tiles.find(kStaticTile);
+	if (!sp || tit == ctx->tiles.end()) return fail(ctx, GCTF_ERR_STATE, "no search plan for Setup1's parameters");
+	const TileTables& tt = *tit->second;
+	if ((sp->ring.n + 15) / 16 * 16 > kStaticMaxRing)
+		return fail(ctx, GCTF_ERR_UNSUPPORTED, "correlation ring of " + std::to_string(sp->ring.n)
+			+ " pixels above the static scratch's " + std::to_string(kStaticMaxRing));
+	if (tt.nbins > kStaticMaxBins) return fail(ctx, GCTF_ERR_UNSUPPORTED, "radius bins above the static scratch");
+	char* base = static_cast<char*>(ctx->static_scratch);
+	SearchState* st = reinterpret_cast<SearchState*>(base + kStStates);
+	ImgPlan* plans = reinterpret_cast<ImgPlan*>(base + kStPlans);
+	float* radial = reinterpret_cast<float*>(base + kStRadial);
+	Search2DView v;
+	v.T = kStaticTile;
+	v.cx = kStaticCx;
+	v.kstride = kStaticMaxRing;
+	v.spect = d_spect;
+	v.plans = plans;
+	v.ring_spec = reinterpret_cast<float*>(base + kStRing);
+	v.ring_ss = reinterpret_cast<double*>(base + kStRingSs);
+	v.st = st;
+	v.part = reinterpret_cast<double2*>(base + kStPart);
+	v.refine = reinterpret_cast<RefineArgs*>(base + kStRefine);
+	v.shell_cc = reinterpret_cast<float*>(base + kStShellCc);
+	v.ctfmap = reinterpret_cast<float*>(base + kStCtfMap);
+	v.bins = reinterpret_cast<double*>(base + kStBins);
+	v.nbins = tt.nbins;
+	v.boff = tt.boff.as<int>();
+	v.bcnt = tt.bcnt.as<int>();
+	v.bidx = tt.bidx.as<int>();
+	const float p0 = ctx->prm.ext_phase_deg, prange = std::max(ctx->prm.ext_phase_range_deg, 0.0f);
+	// the array keeps the previous call's contents: every state starts fresh
+	// (the other buffers are written before they are read)
+	k_state_init<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n, p0, prange);
+	HIPCHK(ctx, hipGetLastError());
+	CHK(fill_plans_at(ctx, plans, n, *sp));
+	CHK(launch_1d_at(ctx, tt, d_spect, radial, plans, st, n, 0, 0.f, 0.f));
+	CHK(run_2d_program(ctx, v, n, NX_BEGIN_DO2D, plain_program(), prange));
+	k_results_out<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n, d_out);
+	HIPCHK(ctx, hipGetLastError());
+	return GCTF_OK;
 }
 
 int gctf_refine(gctf_ctx* ctx, const float* df_mean, const float* ast_ratio, const float* ast_angle_deg,

@@ -24,6 +24,7 @@ EXPORTS = [
     "gctf_create", "gctf_destroy", "gctf_error_string", "gctf_set_stream", "gctf_synchronize",
     "gctf_rescale_setup", "gctf_rescale", "gctf_scaled_image", "gctf_setup1", "gctf_setup2",
     "gctf_set_phase", "gctf_set_search1d", "gctf_gen_half_spectrum", "gctf_do1d", "gctf_refine1d", "gctf_do2d",
+    "gctf_do2d_batch_static",
     "gctf_refine", "gctf_spect_size", "gctf_get_half_spect", "gctf_set_half_spect",
     "gctf_full_spectrum", "gctf_tile_grid", "gctf_ring_pixels", "gctf_radial_average", "gctf_estimate_batch", "gctf_tune", "gctf_profile",
     "gctf_profile_report", "gctf_grid_search_batch",
@@ -137,6 +138,7 @@ def load():
         "gctf_profile_report": (ci, [vp, ci, fp, fp, fp, fp]),
         "gctf_grid_search_batch": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf,
                                         ctypes.POINTER(GctfGrid), fp, fp, fp, fp]),
+        "gctf_do2d_batch_static": (ci, [vp, vp, ci, vp]),
         "gctf_estimate_list": (ci, [vp, ctypes.POINTER(GctfImage), ci, fp]),
         "gctf_estimate_series": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, fp, ci, fp]),
         "gctf_autotune": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, fp, fp, fp]),
@@ -251,6 +253,13 @@ class Context:
         r = GctfResult()
         self.check(self.L.gctf_do2d(self.h, ctypes.byref(r)))
         return r.as_dict()
+
+    def do2d_batch_static(self, dev_spect_ptr, n, dev_out_ptr):
+        """Do1D + Do2D on n <= 16 device half spectra, results into n device
+        gctf_result rows; allocates nothing (static scratch per GPU).
+        Enqueued only: synchronize() before reading the results."""
+        self.check(self.L.gctf_do2d_batch_static(self.h, ctypes.c_void_p(dev_spect_ptr), n,
+                                                 ctypes.c_void_p(dev_out_ptr)))
 
     def refine(self, df_mean, ast_ratio, ast_angle, ext_phase):
         a = [np.asarray(v, np.float32) for v in (df_mean, ast_ratio, ast_angle, ext_phase)]

@@ -106,6 +106,20 @@ GCTF_API int gctf_do2d(gctf_ctx* ctx, gctf_result* out);
  * afExtPhase[2]) (:606-611); [0] = centre, [1] = full range. */
 GCTF_API int gctf_refine(gctf_ctx* ctx, const float* df_mean, const float* ast_ratio,
 	const float* ast_angle_deg, const float* ext_phase_deg, gctf_result* out);
+/* No counterpart in the reference: gctf_do2d for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  CFindCtfBase::SetHalfSpect + Do2D
+ * (:550, :605) -- Do1D, then the 2D search -- on n <= 16 background-removed
+ * half spectra already on the device, d_spect [n][512][257] floats, at the
+ * parameters of the last gctf_setup1 and the -ExtPhase values of
+ * gctf_create; d_out gets n results in device memory.  Tile size 512 only.
+ * Calls no hipMalloc/hipFree: its search buffers live in one static device
+ * array per GPU (about 18 MB, held until the process ends), so calls that
+ * overlap on one GPU, from any context, race.  Returns once the work is
+ * enqueued on the context's stream (gctf_synchronize waits for it).
+ * GCTF_ERR_ARG for n > 16; GCTF_ERR_STATE before gctf_setup1 or while
+ * gctf_profile is on. */
+GCTF_API int gctf_do2d_batch_static(gctf_ctx* ctx, const float* d_spect, int n, gctf_result* d_out);
 /* CFindCtfBase::GetSpectSize(int*, bool bHalf) (:552). */
 GCTF_API int gctf_spect_size(gctf_ctx* ctx, int* size, int half);
 /* CFindCtfBase::GetHalfSpect(bool bRaw, bool bToHost=true) (:551): copies
