@@ -1,7 +1,9 @@
 """The full-size inputs of the benchmark configurations, checked through
 properties that do not depend on the size: a tilt series of 41 micrographs of
-3838x3710 at 1.35 A (the zero-tilt package equals the single-micrograph full
-search and the FP64 oracle; every package recovers its generating defocus),
+3838x3710 at 1.35 A with the tilted specimen's in-plane defocus gradient (the
+zero-tilt package equals the single-micrograph full search and the FP64
+oracle; three later packages equal the oracle's Refine around the same
+centre; every package recovers its generating centre defocus),
 and an uneven list of 4096^2 float32 and 5760x4092 float16 micrographs at 0.8
 and 1.1 A (the list equals the single-micrograph batches), and an
 11520x8184 super-resolution micrograph at 0.5 A (equal to its 2x
@@ -34,11 +36,15 @@ def test_tilt_series_3838x3710_full_size():
     n = len(tilts)
     imgs = torch.empty((n, ny, nx), dtype=torch.float32, device="cuda")
     truth = []
+    span = []
     for i, t in enumerate(tilts):
         dfm = 30000.0 + 2000.0 * math.sin(math.radians(float(t)))
+        # tilted specimen: the defocus changes by pix tan(tilt) A per column
+        ddf = pix * math.tan(math.radians(float(t)))
         synth.micrograph(ny, nx, pix=pix, seed=100 + i, df1=dfm + 200.0, df2=dfm - 200.0, az_deg=45.0,
-                         device="cuda", out=imgs[i])
+                         device="cuda", out=imgs[i], ddf_dx=ddf)
         truth.append(dfm)
+        span.append(abs(ddf) * nx)
     torch.cuda.synchronize()
     ctx = gctf.Context()
     got = ctx.estimate_series(imgs.data_ptr(), n, nx, ny, pix, tilts, tomo=True)
@@ -49,11 +55,36 @@ def test_tilt_series_3838x3710_full_size():
         assert got[i0][k] == one[k], (k, got[i0][k], one[k])
     for i in range(n):
         dm = 0.5 * (got[i]["df_min"] + got[i]["df_max"])
-        assert abs(dm - truth[i]) < 500.0, (i, float(tilts[i]), dm, truth[i])
+        print(f"tilt {float(tilts[i]):6.1f}  defocus span {span[i]:6.0f} A  estimate {dm:8.1f}  truth {truth[i]:8.1f}",
+              flush=True)
+        # the tile average mixes the defocus values of all columns
+        assert abs(dm - truth[i]) < 500.0 + 0.1 * span[i], (i, float(tilts[i]), dm, truth[i])
     ro, _ = O.estimate(imgs[i0].cpu().numpy(), pix)
     assert abs(one["df_min"] - ro["df_min"]) <= 5.0 and abs(one["df_max"] - ro["df_max"]) <= 5.0
     assert abs(one["azimuth"] - ro["azimuth"]) <= 0.5
     assert abs(one["score"] - ro["score"]) <= 1e-4
+    # single packages of the chain against the oracle's Refine around the
+    # same centre: the chain's reference image before each package (the
+    # best score so far, CFindSeriesCtfs.cpp:196-202), taken from the GPU's
+    # own results; tomo: astigmatism and azimuth fixed (:188-194)
+    order = [i0] + [i for i in range(n) if i != i0]
+    ref, center = i0, {}
+    for i in order[1:]:
+        center[i] = ref
+        if got[ref]["score"] < got[i]["score"]:
+            ref = i
+    for i in (order[1], order[2], n - 1):
+        rp = got[center[i]]
+        dm = 0.5 * (rp["df_min"] + rp["df_max"])
+        ast = (rp["df_max"] - rp["df_min"]) * 0.5 / dm
+        fc = O.FindCtf2D(pix=pix)
+        fc.gen_half_spectrum(O.rescale(imgs[i].cpu().numpy(), pix), nx, ny)
+        r = fc.refine(np.float32([dm, max(dm * 0.5, 5000.0 * pix * pix)]), np.float32([ast, 0.0]),
+                      [rp["azimuth"], 0.0], [rp["ext_phase"], 0.0])
+        print(f"package {i} (tilt {float(tilts[i])}) around image {center[i]}: gpu {got[i]} oracle {r}", flush=True)
+        assert abs(got[i]["df_min"] - r["df_min"]) <= 5.0 and abs(got[i]["df_max"] - r["df_max"]) <= 5.0, i
+        assert abs(got[i]["azimuth"] - r["azimuth"]) <= 0.5, i
+        assert abs(got[i]["score"] - r["score"]) <= 1e-4, i
 
 
 def test_uneven_full_size_list():
