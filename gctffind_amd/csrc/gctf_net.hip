@@ -226,11 +226,12 @@ __global__ __launch_bounds__(256) void k_fc2(const float* __restrict__ h, const 
 }
 
 // Loss per spectrum and its gradient (divided by B).  target[b] =
-// (mean defocus A, astigmatism A, azimuth deg, unused).
+// (mean defocus A, astigmatism A, azimuth deg, confidence label).
 //   100 (o0 - ln(df / 1e4))^2 + 0.25 (o1 - ast / 1e3)^2
-//   + 0.05 (1 - cos 2 (o2 - az)) + 0.5 BCE(sigmoid(o3), c)
-// with c = 1 when |1e4 exp(o0) - df| <= 200 A (the proposal would put the
-// truth well inside a +-300 A window), else 0.
+//   + 0.05 (1 - cos 2 (o2 - az)) + 0.5 w BCE(sigmoid(o3), c)
+// The label is measured on the micrograph (whether the learned path's
+// narrowed search ended within 20 A of the full search): > 0 gives c = 1,
+// < 0 gives c = 0, and 0 (unlabelled) drops the confidence term (w = 0).
 __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ o, const float* __restrict__ tg, int B,
 	float* __restrict__ dO, double* __restrict__ lpart)
 {
@@ -242,20 +243,20 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ o, const
 	const float az = tb[2] * 0.017453292519943295f;
 	const float e0 = ob[0] - y0, e1 = ob[1] - y1;
 	const float d2 = 2.0f * (ob[2] - az);
-	const float pdf = 1.0e4f * expf(ob[0]);
-	const float c = fabsf(pdf - tb[0]) <= 200.0f ? 1.0f : 0.0f;
+	const float c = tb[3] > 0.0f ? 1.0f : 0.0f;
+	const float w = tb[3] != 0.0f ? 1.0f : 0.0f;
 	const float x = ob[3];
 	// softplus forms of the log-sigmoid terms
 	const float sp_pos = fmaxf(x, 0.f) + log1pf(expf(-fabsf(x)));   // -log(1 - p)
 	const float sp_neg = sp_pos - x;                                 // -log p
 	const float bce = c * sp_neg + (1.0f - c) * sp_pos;
 	const float p = 1.0f / (1.0f + expf(-x));
-	const float L = 100.0f * e0 * e0 + 0.25f * e1 * e1 + 0.05f * (1.0f - cosf(d2)) + 0.5f * bce;
+	const float L = 100.0f * e0 * e0 + 0.25f * e1 * e1 + 0.05f * (1.0f - cosf(d2)) + 0.5f * w * bce;
 	const float inv = 1.0f / (float)B;
 	dO[b * kNetOut + 0] = 200.0f * e0 * inv;
 	dO[b * kNetOut + 1] = 0.5f * e1 * inv;
 	dO[b * kNetOut + 2] = 0.1f * sinf(d2) * inv;
-	dO[b * kNetOut + 3] = 0.5f * (p - c) * inv;
+	dO[b * kNetOut + 3] = 0.5f * w * (p - c) * inv;
 	lpart[b] = (double)L;
 }
 

@@ -138,19 +138,43 @@ def draw_params(rng, n, df_lo=3000.0, df_hi=50000.0, ast_max=1500.0, snr=(0.03, 
     return dfm + 0.5 * ast, dfm - 0.5 * ast, az, b, s
 
 
-def generate(torch, ctx, n_total, size=3328, pix=1.2, batch=16, seed0=10_000_000, log=None, budget_s=None):
+AGREE_A = 20.0   # the learned path's acceptance: |mean defocus - plain search's| <= 20 A
+
+
+def agreement_labels(ctx, ptr, m, size, pix, window=300.0, agree_A=AGREE_A):
+    """+1 / -1 per micrograph: whether the learned path without its
+    confidence fallback (the network installed in ctx proposes, the
+    narrowed search decides; window-edge and score <= 0 fallbacks stay on)
+    ends within agree_A of the plain search's mean defocus on the same
+    micrograph."""
+    plain = ctx.estimate_batch(ptr, m, size, size, pix)
+    learned = ctx.estimate_batch_learned(ptr, m, size, size, pix, conf_threshold=0.0, window=window)
+    mp = np.array([0.5 * (r["df_min"] + r["df_max"]) for r in plain])
+    ml = np.array([0.5 * (r["df_min"] + r["df_max"]) for r in learned])
+    return np.where(np.abs(ml - mp) <= agree_A, 1.0, -1.0).astype(np.float32)
+
+
+def generate(torch, ctx, n_total, size=3328, pix=1.2, batch=16, seed0=10_000_000, log=None, budget_s=None,
+             label_every=0, label_params=None):
     """Radial spectra [n][257] (device) and targets [n][4] (device) of
     n_total seeded synthetic micrographs: the batch starting at sample i
     uses seed seed0 + i.  With budget_s, generation stops after the batch
-    that crosses the time budget and the first n generated are returned."""
+    that crosses the time budget and the first n generated are returned.
+    With label_every = k > 0 and label_params (the network parameters that
+    propose), every k-th batch is also searched by the plain and the learned
+    path, and target[:, 3] holds agreement_labels there (0 elsewhere: no
+    label, so those spectra train the proposal but not the confidence)."""
     cx = 257
     radial = torch.empty((n_total, cx), dtype=torch.float32, device="cuda:0")
     target = torch.zeros((n_total, 4), dtype=torch.float32, device="cuda:0")
     imgs = torch.empty((batch, size, size), dtype=torch.float32, device="cuda:0")
     # one staging buffer: the library caches a graph per output address
     stage = torch.empty((batch, cx), dtype=torch.float32, device="cuda:0")
+    if label_every:
+        ctx.set_proposal_params(label_params)
     t0 = time.time()
     last = t0
+    nlab = nagree = 0
     for i0 in range(0, n_total, batch):
         m = min(batch, n_total - i0)
         rng = np.random.default_rng(seed0 + i0)
@@ -159,11 +183,17 @@ def generate(torch, ctx, n_total, size=3328, pix=1.2, batch=16, seed0=10_000_000
         torch.cuda.synchronize()
         ctx.radial_batch(imgs.data_ptr(), m, size, size, pix, stage.data_ptr(), cx)
         radial[i0:i0 + m] = stage[:m]
-        tg = np.stack([0.5 * (d1 + d2), np.abs(d1 - d2), az, np.zeros(m)], 1).astype(np.float32)
+        lab = np.zeros(m, np.float32)
+        if label_every and (i0 // batch) % label_every == 0:
+            lab = agreement_labels(ctx, imgs.data_ptr(), m, size, pix)
+            nlab += m
+            nagree += int((lab > 0).sum())
+        tg = np.stack([0.5 * (d1 + d2), np.abs(d1 - d2), az, lab], 1).astype(np.float32)
         target[i0:i0 + m] = torch.from_numpy(tg).cuda()
         if log and time.time() - last > 30:
             last = time.time()
-            log(f"generated {i0 + m}/{n_total} spectra, {(i0 + m) / (last - t0):.0f}/s")
+            log(f"generated {i0 + m}/{n_total} spectra, {(i0 + m) / (last - t0):.0f}/s"
+                + (f", {nlab} labelled ({nagree} agree)" if label_every else ""))
         if budget_s is not None and time.time() - t0 > budget_s:
             torch.cuda.synchronize()
             return radial[:i0 + m], target[:i0 + m]
@@ -242,6 +272,78 @@ def evaluate_spectra(net, radial, target, batch):
             "frac_within_300A_when_confident": round(float(np.mean(err[ok] <= 300)) if ok.any() else 0.0, 4)}
 
 
+# ---- confidence threshold ----------------------------------------------------
+def wide_params(rng):
+    """One micrograph of the held-out distribution: mean defocus
+    log-uniform 3000-50000 A, astigmatism 0-1500 A, B 50-200 A^2, SNR
+    log-uniform 0.03-0.3, azimuth uniform in [-90, 90), drawn in that order.
+    Returns (df1, df2, azimuth, B, SNR)."""
+    dfm = float(math.exp(rng.uniform(math.log(3000.0), math.log(50000.0))))
+    ast = float(rng.uniform(0.0, 1500.0))
+    bfac, snr = float(rng.uniform(50.0, 200.0)), float(math.exp(rng.uniform(math.log(0.03), math.log(0.3))))
+    az = float(rng.uniform(-90.0, 90.0))
+    return dfm + ast / 2, dfm - ast / 2, az, bfac, snr
+
+
+def wide_batch(torch, n, size, pix, seed0, out):
+    """n micrographs of the held-out distribution (micrograph i from seed
+    seed0 + i, gctffind_amd.synth's image model) into out[:n]; returns the
+    generating mean defocus values."""
+    from . import synth
+    truth = []
+    for i in range(n):
+        d1, d2, az, b, snr = wide_params(np.random.default_rng(seed0 + i))
+        synth.micrograph(size, size, pix=pix, seed=seed0 + i, df1=d1, df2=d2, az_deg=az, bfac=b, snr=snr,
+                         device="cuda:0", out=out[i])
+        truth.append(0.5 * (d1 + d2))
+    torch.cuda.synchronize()
+    return np.array(truth)
+
+
+THRESHOLDS = (0.0, 0.5, 0.6, 0.7, 0.8, 0.85, 0.9, 0.95, 0.98)
+
+
+def calibrate(torch, ctx, params, n=1024, seed0=5000, size=4096, pix=1.0, batch=32, target=0.99,
+              thresholds=THRESHOLDS, log=None):
+    """The confidence threshold on a tuning set (seeds seed0 .., separate
+    from the evaluation seeds): per micrograph, the learned path without
+    its confidence fallback against the plain search.  A micrograph below
+    the threshold takes the plain search's result, so it agrees by
+    construction.  Returns the table over `thresholds` (fraction within
+    AGREE_A, fallback fraction) and the lowest threshold whose fraction
+    reaches `target` (None if none does)."""
+    ctx.set_proposal_params(params)
+    buf = torch.empty((batch, size, size), dtype=torch.float32, device="cuda:0")
+    agree, conf, edge = [], [], []
+    for i0 in range(0, n, batch):
+        m = min(batch, n - i0)
+        wide_batch(torch, m, size, pix, seed0 + i0, buf)
+        lr, props = ctx.estimate_batch_learned(buf.data_ptr(), batch, size, size, pix, conf_threshold=0.0,
+                                               proposals=True)
+        pr = ctx.estimate_batch(buf.data_ptr(), batch, size, size, pix)
+        for j in range(m):
+            ml = 0.5 * (lr[j]["df_min"] + lr[j]["df_max"])
+            mp = 0.5 * (pr[j]["df_min"] + pr[j]["df_max"])
+            agree.append(abs(ml - mp) <= AGREE_A)
+            conf.append(props[j]["confidence"])
+            edge.append(bool(props[j]["fallback"]))
+        if log:
+            log(f"calibration {i0 + m}/{n}")
+    agree, conf, edge = np.array(agree), np.array(conf), np.array(edge)
+    table, chosen = [], None
+    for t in thresholds:
+        fb = edge | (conf < t)
+        ok = agree | fb
+        row = {"threshold": t, "frac_within_20A": round(float(ok.mean()), 4),
+               "fallback_frac": round(float(fb.mean()), 4)}
+        table.append(row)
+        if chosen is None and row["frac_within_20A"] >= target:
+            chosen = t
+    return {"n": int(n), "seeds": f"{seed0}..{seed0 + n - 1}", "micrograph": f"{size}x{size} at {pix} A",
+            "target": target, "table": table, "threshold": chosen,
+            "frac_agree_without_fallback": round(float(agree.mean()), 4)}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("cmd", choices=["train"])
@@ -255,17 +357,36 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--gen-budget", type=float, default=None, help="seconds of spectrum generation at most")
+    ap.add_argument("--label-every", type=int, default=4,
+                    help="every k-th generated batch also gets agreement labels for the confidence (0: none)")
+    ap.add_argument("--label-weights", default=DEFAULT_WEIGHTS,
+                    help="network that proposes while the agreement labels are measured")
+    ap.add_argument("--calibrate", type=int, default=1024,
+                    help="tuning micrographs (4096^2 at 1.0 A, seeds 5000..) for the confidence threshold (0: skip)")
     a = ap.parse_args(argv)
     import torch
     log = lambda s: print(s, flush=True)
     ctx = gctf.Context()
     t0 = time.time()
-    radial, target = generate(torch, ctx, a.n, a.size, a.pix, a.gen_batch, log=log, budget_s=a.gen_budget)
+    lab_params = load_params(a.label_weights)[0] if a.label_every else None
+    radial, target = generate(torch, ctx, a.n, a.size, a.pix, a.gen_batch, log=log, budget_s=a.gen_budget,
+                              label_every=a.label_every, label_params=lab_params)
     gen_s = time.time() - t0
-    log(f"generated {radial.shape[0]} spectra in {gen_s:.0f} s")
+    lab = target[:, 3].cpu().numpy()
+    log(f"generated {radial.shape[0]} spectra in {gen_s:.0f} s, {int((lab != 0).sum())} with agreement labels")
     params, rep = train(torch, radial, target, a.epochs, a.batch, a.lr, seed=a.seed, log=log)
     meta = {"spectra": int(radial.shape[0]), "micrograph": f"{a.size}x{a.size} at {a.pix} A", "seed0": 10_000_000,
-            "generation_seconds": round(gen_s, 1), **rep}
+            "generation_seconds": round(gen_s, 1),
+            "confidence_labels": {"n": int((lab != 0).sum()), "frac_agree": round(float((lab > 0).sum() / max(1, (lab != 0).sum())), 4),
+                                  "rule": f"learned path without the confidence fallback within {AGREE_A:.0f} A of the "
+                                          f"plain search, proposals of {os.path.basename(a.label_weights)}"},
+            **rep}
+    del radial, target
+    torch.cuda.empty_cache()
+    if a.calibrate:
+        cal = calibrate(torch, ctx, params, n=a.calibrate, log=log)
+        meta["calibration"] = cal
+        meta["conf_threshold"] = cal["threshold"] if cal["threshold"] is not None else 0.8
     save_params(a.out, params, meta)
     log(json.dumps({k: v for k, v in meta.items() if k != "loss_curve"}))
     return 0

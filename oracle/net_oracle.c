@@ -15,8 +15,9 @@
  *   fc2     128 -> 4
  *   loss    mean over the batch of
  *           100 (o0 - ln(df / 1e4))^2 + 0.25 (o1 - ast / 1e3)^2
- *           + 0.05 (1 - cos 2 (o2 - az)) + 0.5 BCE(sigmoid(o3), c),
- *           c = [|1e4 exp(o0) - df| <= 200] (held constant)
+ *           + 0.05 (1 - cos 2 (o2 - az)) + 0.5 w BCE(sigmoid(o3), c),
+ *           c = 1 if target[3] > 0, 0 if < 0; w = 0 when target[3] == 0
+ *           (no confidence label), else 1
  *
  * Parity pin: the gradient of orc_net_loss_grad is checked against central
  * finite differences of orc_net_loss_grad's own loss (tests/test_net.py),
@@ -125,7 +126,9 @@ static double loss1(const double* o, const float* tg, double* d)
 {
 	const double y0 = log(tg[0] * 1e-4), y1 = tg[1] * 1e-3, az = tg[2] * (M_PI / 180.0);
 	const double e0 = o[0] - y0, e1 = o[1] - y1, d2 = 2.0 * (o[2] - az);
-	const double c = fabs(1e4 * exp(o[0]) - tg[0]) <= 200.0 ? 1.0 : 0.0;
+	/* label sign: > 0 agrees (c = 1), < 0 disagrees (c = 0), 0 no label (w = 0) */
+	const double c = tg[3] > 0.0f ? 1.0 : 0.0;
+	const double w = tg[3] != 0.0f ? 1.0 : 0.0;
 	const double x = o[3];
 	const double sp_pos = (x > 0 ? x : 0.0) + log1p(exp(-fabs(x)));
 	const double sp_neg = sp_pos - x;
@@ -134,8 +137,8 @@ static double loss1(const double* o, const float* tg, double* d)
 	d[0] = 200.0 * e0;
 	d[1] = 0.5 * e1;
 	d[2] = 0.1 * sin(d2);
-	d[3] = 0.5 * (p - c);
-	return 100.0 * e0 * e0 + 0.25 * e1 * e1 + 0.05 * (1.0 - cos(d2)) + 0.5 * bce;
+	d[3] = 0.5 * w * (p - c);
+	return 100.0 * e0 * e0 + 0.25 * e1 * e1 + 0.05 * (1.0 - cos(d2)) + 0.5 * w * bce;
 }
 
 /* backward of conv (cin -> CH, length L): dA_out [CH][L/2] -> grads and dIn [cin][L] */

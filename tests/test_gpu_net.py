@@ -35,8 +35,9 @@ def _inputs(n, seed):
     rng = np.random.default_rng(seed)
     x = np.linspace(0, 1, 257, dtype=np.float32)
     r = (rng.normal(size=(n, 257)) * 0.5 + np.sin(40 * x[None] ** 2 * rng.uniform(0.5, 2, (n, 1)))).astype(np.float32)
+    # confidence labels of every kind: agrees (+1), disagrees (-1), unlabelled (0)
     tg = np.stack([np.exp(rng.uniform(np.log(3000), np.log(50000), n)), rng.uniform(0, 1500, n),
-                   rng.uniform(-90, 90, n), np.zeros(n)], 1).astype(np.float32)
+                   rng.uniform(-90, 90, n), rng.choice([-1.0, 0.0, 1.0], n)], 1).astype(np.float32)
     return r, tg
 
 

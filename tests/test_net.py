@@ -30,15 +30,20 @@ def test_loss_closed_form():
     o = np.array([0.4, 0.3, 0.2, -0.7])
     p[b6] = o
     r, tg = _inputs(4)
+    tg[:, 3] = [1.0, -1.0, 0.0, 1.0]        # agrees, disagrees, no label, agrees
     L = O.net_loss_grad(p, r, tg, grad=False)
     want = 0.0
     for t in tg.astype(np.float64):
-        c = 1.0 if abs(1e4 * math.exp(o[0]) - t[0]) <= 200 else 0.0
+        c, w = (1.0 if t[3] > 0 else 0.0), (1.0 if t[3] != 0 else 0.0)
         pr = 1 / (1 + math.exp(-o[3]))
         want += (100 * (o[0] - math.log(t[0] / 1e4)) ** 2 + 0.25 * (o[1] - t[1] / 1e3) ** 2
                  + 0.05 * (1 - math.cos(2 * (o[2] - math.radians(t[2]))))
-                 - 0.5 * (c * math.log(pr) + (1 - c) * math.log(1 - pr)))
+                 - 0.5 * w * (c * math.log(pr) + (1 - c) * math.log(1 - pr)))
     assert L == pytest.approx(want / 4, rel=1e-12)
+    # d loss / d (confidence bias) = sum of 0.5 w (p - c) / n
+    _, g = O.net_loss_grad(p, r, tg)
+    pr = 1 / (1 + math.exp(-o[3]))
+    assert g[b6.start + 3] == pytest.approx(0.5 * ((pr - 1) + pr + 0.0 + (pr - 1)) / 4, rel=1e-9)
     out = O.net_forward(p, r)
     assert np.array_equal(out, np.tile(o, (4, 1)))
 
@@ -64,6 +69,7 @@ def test_oracle_gradient_finite_differences():
     for _, bs in proposal.layer_slices().values():
         p[bs] += rng0.normal(0.0, 0.05, bs.stop - bs.start)
     r, tg = _inputs(3, 2)
+    tg[:, 3] = [1.0, -1.0, 0.0]
     L, g = O.net_loss_grad(p, r, tg)
     rng = np.random.default_rng(7)
     h = 1e-6
