@@ -109,6 +109,8 @@ def parse():
     ap.add_argument("--eval", type=int, default=1024,
                     help="learned mode: held-out micrographs for the defocus agreement (0: skip)")
     ap.add_argument("--weights", default=None, help="proposal-network weights (.npz)")
+    ap.add_argument("--conf-threshold", type=float, default=None,
+                    help="learned path: fallback below this confidence (default: the weights' calibrated one)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip gctf_autotune (images per chunk x tile splits chosen by timing)")
     ap.add_argument("--chunk", type=int, default=0, help="images per spectrum chunk (0: library default)")
@@ -154,17 +156,17 @@ def make_batch(torch, n, size, pix, seed0=1000, out=None, wide=False):
     import numpy as np
     imgs = out if out is not None else torch.empty((n, size, size), dtype=torch.float32, device="cuda:0")
     truth = []
+    from gctffind_amd import proposal
     for i in range(n):
         rng = np.random.default_rng(seed0 + i)
         if wide:
-            dfm = float(math.exp(rng.uniform(math.log(3000.0), math.log(50000.0))))
-            ast = float(rng.uniform(0.0, 1500.0))
-            bfac, snr = float(rng.uniform(50.0, 200.0)), float(math.exp(rng.uniform(math.log(0.03), math.log(0.3))))
+            d1, d2, az, bfac, snr = proposal.wide_params(rng)
+            dfm, ast = 0.5 * (d1 + d2), d1 - d2
         else:
             dfm = float(math.exp(rng.uniform(math.log(8000.0), math.log(30000.0))))
             ast = float(rng.uniform(0.0, 1000.0))
             bfac, snr = 100.0, 0.1
-        az = float(rng.uniform(-90.0, 90.0))
+            az = float(rng.uniform(-90.0, 90.0))
         synth.micrograph(size, size, pix=pix, seed=seed0 + i, df1=dfm + ast / 2, df2=dfm - ast / 2,
                          az_deg=az, bfac=bfac, snr=snr, device="cuda:0", out=imgs[i])
         truth.append((dfm, ast, az))
@@ -194,6 +196,8 @@ def main():
     if learned:
         from gctffind_amd import proposal
         params, meta = proposal.load_params(a.weights or proposal.DEFAULT_WEIGHTS)
+        # the confidence threshold calibrated on the tuning seeds with the weights
+        thr = float(meta.get("conf_threshold", 0.8)) if a.conf_threshold is None else a.conf_threshold
         ctx.set_proposal_params(params)
     imgs, truth = make_batch(torch, n, a.size, a.pix, seed0=2000 if learned else 1000, wide=learned)
     ptr = imgs.data_ptr()
@@ -206,7 +210,7 @@ def main():
     if a.mode == "grid":
         run = lambda: ctx.grid_search_batch(ptr, n, a.size, a.size, a.pix, grid)
     elif learned:
-        run = lambda: ctx.estimate_batch_learned(ptr, n, a.size, a.size, a.pix)
+        run = lambda: ctx.estimate_batch_learned(ptr, n, a.size, a.size, a.pix, conf_threshold=thr)
     else:
         run = plain_run
     log(f"timing the {a.mode} path: batch {n}, {a.steps} steps after {a.warmup} warm-up")
@@ -253,7 +257,13 @@ def main():
             out["config"]["plain_value"] = round(pv, 2)
             out["config"]["plain_ms_per_step"] = round(pms, 3)
             out["config"]["learned_over_plain"] = round(value / pv, 3)
-        out["config"]["proposal_weights"] = {k: meta.get(k) for k in ("spectra", "micrograph", "epochs", "steps")}
+        out["config"]["proposal_weights"] = {k: meta.get(k) for k in ("spectra", "micrograph", "epochs", "steps",
+                                                                      "confidence_labels")}
+        out["config"]["conf_threshold"] = thr
+        if "calibration" in meta:
+            out["config"]["conf_calibration"] = {k: meta["calibration"][k] for k in
+                                                 ("seeds", "micrograph", "target", "threshold", "table",
+                                                  "frac_agree_without_fallback")}
     dfe = [abs(0.5 * (r["df_min"] + r["df_max"]) - t[0]) for r, t in zip(res, truth)]
     out["config"]["median_abs_ddf_vs_truth_A"] = round(float(sorted(dfe)[len(dfe) // 2]), 1)
 
@@ -269,7 +279,7 @@ def main():
             pres = ctx.estimate_batch(ptr, n, a.size, a.size, a.pix)
     if learned and a.eval:
         log(f"evaluating {a.eval} held-out micrographs")
-        out["config"]["eval"] = evaluate(torch, ctx, a, imgs)
+        out["config"]["eval"] = evaluate(torch, ctx, a, imgs, thr)
     del imgs
     torch.cuda.empty_cache()
     if a.grid_leg:
@@ -343,7 +353,7 @@ def uneven_list(torch, gctf, ctx, n, steps, warmup):
             "median_abs_ddf_vs_truth_A": round(float(dfe[len(dfe) // 2]), 1)}
 
 
-def evaluate(torch, ctx, a, buf):
+def evaluate(torch, ctx, a, buf, thr):
     """Learned path against the plain path (the reference algorithm; its
     defocus and azimuth equal the FP64 oracle's on the parity cases) on a.eval held-out micrographs,
     seeds 2000..: mean-defocus agreement overall, inside the reference's 1D
@@ -360,7 +370,8 @@ def evaluate(torch, ctx, a, buf):
     for i0 in range(0, a.eval, n):
         m = min(n, a.eval - i0)
         _, truth = make_batch(torch, m, a.size, a.pix, seed0=2000 + i0, out=buf, wide=True)
-        lr, props = ctx.estimate_batch_learned(buf.data_ptr(), n, a.size, a.size, a.pix, proposals=True)
+        lr, props = ctx.estimate_batch_learned(buf.data_ptr(), n, a.size, a.size, a.pix, proposals=True,
+                                               conf_threshold=thr)
         pr = ctx.estimate_batch(buf.data_ptr(), n, a.size, a.size, a.pix)
         for j in range(m):
             rows.append((truth[j][0], 0.5 * (lr[j]["df_min"] + lr[j]["df_max"]), 0.5 * (pr[j]["df_min"] + pr[j]["df_max"]),
@@ -669,9 +680,10 @@ def grid_leg(torch, gctf, a):
 
 def series_leg(torch, gctf, n, steps, warmup):
     """A tilt series of n 3838x3710 float32 micrographs at 1.35 A, tilts
-    -60..60 degrees, defocus 30000 +- 2000 sin(tilt) A with astigmatism 400 A
-    at 45 degrees, through gctf_estimate_series (the reference's package
-    chain on the device, tomo mode)."""
+    -60..60 degrees, defocus 30000 +- 2000 sin(tilt) A at the centre with a
+    linear in-plane gradient of pix tan(tilt) A per column, astigmatism
+    400 A at 45 degrees, through gctf_estimate_series (the reference's
+    package chain on the device, tomo mode)."""
     import numpy as np
     from gctffind_amd import synth
     nx, ny, pix = 3838, 3710, 1.35
@@ -681,7 +693,7 @@ def series_leg(torch, gctf, n, steps, warmup):
     for i, t in enumerate(tilts):
         dfm = 30000.0 + 2000.0 * math.sin(math.radians(float(t)))
         synth.micrograph(ny, nx, pix=pix, seed=100 + i, df1=dfm + 200.0, df2=dfm - 200.0, az_deg=45.0,
-                         device="cuda:0", out=imgs[i])
+                         device="cuda:0", out=imgs[i], ddf_dx=pix * math.tan(math.radians(float(t))))
         truth.append(dfm)
     torch.cuda.synchronize()
     ctx = gctf.Context()
@@ -690,6 +702,7 @@ def series_leg(torch, gctf, n, steps, warmup):
     dfe = np.array([abs(0.5 * (r["df_min"] + r["df_max"]) - t) for r, t in zip(res, truth)])
     del imgs
     return {"micrographs": n, "shape": f"{nx}x{ny} f32 at {pix} A", "tilts": "-60..60 degrees",
+            "in_plane_gradient": "pix tan(tilt) A per column (defocus span up to 8990 A at 60 degrees)",
             "value": round(value, 2), "unit": "micrographs/s", "ms_per_series": round(ms, 3),
             "median_abs_ddf_vs_truth_A": round(float(np.median(dfe)), 1),
             "max_abs_ddf_vs_truth_A": round(float(dfe.max()), 1)}

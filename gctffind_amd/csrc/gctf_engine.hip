@@ -183,8 +183,10 @@ constexpr size_t kStPart = kStBins + st_align(sizeof(double) * 6 * kStaticMaxBin
 constexpr size_t kStRing = kStPart
 	+ st_align(sizeof(double2) * (size_t)kMaxCands * kPixChunks * kStaticMaxImages);
 constexpr size_t kStCtfMap = kStRing + st_align(sizeof(float) * (size_t)kStaticMaxRing * kStaticMaxImages);
-constexpr size_t kStaticScratchBytes = kStCtfMap
-	+ st_align(sizeof(float) * (size_t)kStaticTile * kStaticCx * kStaticMaxImages);
+// 1D search: the workgroup maxima of the reference's 501-point scan x 37 phases
+constexpr int kStaticS1dLd = (501 * 37 + kS1dPerWg - 1) / kS1dPerWg;
+constexpr size_t kStS1d = kStCtfMap + st_align(sizeof(float) * (size_t)kStaticTile * kStaticCx * kStaticMaxImages);
+constexpr size_t kStaticScratchBytes = kStS1d + st_align(sizeof(float2) * (size_t)kStaticS1dLd * kStaticMaxImages);
 
 }  // namespace
 
@@ -226,6 +228,9 @@ struct gctf_ctx {
 	gctf_params prm{};
 	std::string err;
 	int chunk = 4, tile_splits = 8, use_graph = 1;
+	int cpb = 8;          // candidates per 2D-evaluator workgroup (search blocking: 4, 8 or 16)
+	int list_queue = 1;   // tile passes of a list: 1 persistent workgroups on a work queue, 0 one item each
+	int tune_gen = 0;     // bumped by gctf_autotune (list graphs read the tuned values)
 	float scan_lo = 0, scan_hi = 0, scan_step = 0;  // Do1D scan (gctf_set_search1d; 0: reference)
 	int gen = 0;  // bumped whenever a device buffer moves
 
@@ -233,7 +238,8 @@ struct gctf_ctx {
 	std::map<int, std::unique_ptr<TileTables>> tiles;
 	std::map<std::string, std::unique_ptr<SearchPlan>> splans;
 	std::map<std::string, GraphEntry> graphs;
-	std::map<std::string, std::pair<int, int>> tuned;  // shape -> (chunk, tile splits) from gctf_autotune
+	struct Tuned { int chunk, splits, cpb; };
+	std::map<std::string, Tuned> tuned;  // shape -> gctf_autotune's choice
 
 	// single-image (reference-style) state
 	RescalePlan rp;
@@ -246,7 +252,9 @@ struct gctf_ctx {
 	// batch buffers
 	int cap = 0;  // images the per-image buffers hold
 	DevBuf cbufA, cbufB, padc, bsums, rowsum, rt, tpart;
+	DevBuf s1part; // 1D search: workgroup maxima of a brute-force stage
 	DevBuf jobs;   // tile work lists of gctf_estimate_list
+	DevBuf rjobs, rtab, list_plans, qnext;   // its rescale jobs, class tables, plans, work-queue counters
 	DevBuf raw, hsum, spect, mom, q, radial, ring_spec, ring_ss, states, part2d, refine, shell_cc, fullsp,
 		moments1, plans, shell_bins;
 	int kstride = 0;  // ring pixels reserved per image in ring_spec
@@ -655,9 +663,23 @@ int check_tile(gctf_ctx* c, int T)
 	return GCTF_OK;
 }
 
+// Workgroup maxima per image of a 1D brute-force stage: the largest stage
+// is the initial scan (the reference's 501 points, or the configured scan)
+// times 37 phases, or the refine stage (at most 501 x 37).
+// Workgroups of a 1D stage with at most nps phases.
+int s1d_groups(const gctf_ctx* c, int nps)
+{
+	long long ndf = 501;
+	if (c->scan_step > 0.0f)
+		ndf = std::max(ndf, (long long)((double)(c->scan_hi - c->scan_lo) / (double)c->scan_step) + 3);
+	return (int)((ndf * nps + kS1dPerWg - 1) / kS1dPerWg);
+}
+int s1d_part_ld(const gctf_ctx* c) { return s1d_groups(c, 37); }
+
 // Per-image buffers for `n` images of tile size T.
 int ensure_image_buffers(gctf_ctx* c, int n, int T, int ring_n)
 {
+	CHK(ensure_buf(c, c->s1part, sizeof(float2) * (size_t)s1d_part_ld(c) * n));
 	const size_t cx = T / 2 + 1, npix = cx * T;
 	const int nparts = (int)((npix + 255) / 256);
 	CHK(ensure_buf(c, c->raw, sizeof(float) * npix * n));
@@ -922,9 +944,11 @@ int fill_plans(gctf_ctx* c, int i0, int m, const SearchPlan& sp)
 }
 
 // Radial average + CFindCtf1D::Do1D / Refine1D for m images: spectra
-// [m][T][T/2+1], radial averages [m][T/2+1], plans and states [m].
+// [m][T][T/2+1], radial averages [m][T/2+1], plans and states [m]; part
+// [m][ldp] holds the workgroup maxima of a brute-force stage
+// (ldp >= s1d_part_ld(c)).
 int launch_1d_at(gctf_ctx* c, const TileTables& tt, const float* spect, float* radial, const ImgPlan* plans,
-	SearchState* st, int m, int mode, float init_df, float range)
+	SearchState* st, int m, int mode, float init_df, float range, float2* part, int ldp, int groups)
 {
 	hipStream_t s = c->stream;
 	const int T = tt.T, cx = T / 2 + 1;
@@ -934,17 +958,29 @@ int launch_1d_at(gctf_ctx* c, const TileTables& tt, const float* spect, float* r
 	p.cmpx = cx;
 	p.mode = mode; p.init_df = init_df; p.range = range;
 	p.scan_lo = c->scan_lo; p.scan_hi = c->scan_hi; p.scan_step = c->scan_step;
-	PROF(c, "k_search1d", k_search1d<<<m, 256, 0, s>>>(p, plans, radial, tt.w1d.as<double>(), st));
+	PROF(c, "k_s1d_begin", k_s1d_begin<<<(m + 63) / 64, 64, 0, s>>>(p, plans, st, m));
+	// Do1D: the scan, then mRefineDefocus around its best; Refine1D: the latter only
+	for (int round = 0; round < (mode == 0 ? 2 : 1); ++round) {
+		PROF(c, "k_s1d_eval", k_s1d_eval<<<dim3(groups, m), 256, 0, s>>>(cx, plans, radial, tt.w1d.as<double>(), st,
+			ldp, part));
+		PROF(c, "k_s1d_pick", k_s1d_pick<<<m, 256, 0, s>>>(st, plans, ldp, part));
+	}
 	HIPCHK(c, hipGetLastError());
 	return GCTF_OK;
 }
 
-// Radial average + CFindCtf1D::Do1D / Refine1D for images [i0, i0 + m).
-int launch_1d(gctf_ctx* c, const TileTables& tt, int i0, int m, int mode, float init_df, float range)
+// Radial average + CFindCtf1D::Do1D / Refine1D for images [i0, i0 + m);
+// phase_search: the states may hold a phase range (37 phase steps).
+int launch_1d(gctf_ctx* c, const TileTables& tt, int i0, int m, int mode, float init_df, float range,
+	bool phase_search = true)
 {
 	const int cx = tt.T / 2 + 1;
+	const int ldp = s1d_part_ld(c);
+	if ((size_t)(i0 + m) * ldp * sizeof(float2) > c->s1part.bytes)
+		return fail(c, GCTF_ERR_STATE, "1D search buffer smaller than the configured scan");
 	return launch_1d_at(c, tt, c->spect.as<float>() + (size_t)i0 * tt.T * cx, c->radial.as<float>() + (size_t)i0 * cx,
-		c->plans.as<ImgPlan>() + i0, c->states.as<SearchState>() + i0, m, mode, init_df, range);
+		c->plans.as<ImgPlan>() + i0, c->states.as<SearchState>() + i0, m, mode, init_df, range,
+		c->s1part.as<float2>() + (size_t)i0 * ldp, ldp, s1d_groups(c, phase_search ? 37 : 1));
 }
 
 // Device pointers of the 2D search for images [i0, i0 + m).
@@ -998,11 +1034,17 @@ int run_2d_program(gctf_ctx* c, const Search2DView& v, int m, int begin, const S
 	hipStream_t s = c->stream;
 	PROF(c, "k_ring_gather", k_ring_gather<<<m, 1024, 0, s>>>(v.spect, v.T * v.cx, v.plans, v.kstride,
 		v.ring_spec, v.ring_ss));
-	constexpr int CPB = 8;
+	// search blocking: candidates per workgroup (each candidate's sums do not
+	// depend on its neighbours, so the choice leaves the results unchanged)
 	auto eval = [&](int maxc) {
-		dim3 grid((maxc + CPB - 1) / CPB, kPixChunks, m);
-		PROF(c, "k_cc2d_eval", k_cc2d_eval<CPB><<<grid, 256, 0, s>>>(v.plans, v.ring_spec, v.kstride, v.st,
-			v.part));
+		const int cpb = c->cpb;
+		dim3 grid((maxc + cpb - 1) / cpb, kPixChunks, m);
+		if (cpb == 4)
+			PROF(c, "k_cc2d_eval", k_cc2d_eval<4><<<grid, 256, 0, s>>>(v.plans, v.ring_spec, v.kstride, v.st, v.part));
+		else if (cpb == 16)
+			PROF(c, "k_cc2d_eval", k_cc2d_eval<16><<<grid, 256, 0, s>>>(v.plans, v.ring_spec, v.kstride, v.st, v.part));
+		else
+			PROF(c, "k_cc2d_eval", k_cc2d_eval<8><<<grid, 256, 0, s>>>(v.plans, v.ring_spec, v.kstride, v.st, v.part));
 	};
 	auto step = [&](int next) {
 		PROF(c, "k_s2d_step", k_s2d_step<<<m, 256, 0, s>>>(v.st, v.part, v.ring_ss, next, v.refine, pg));
@@ -1295,12 +1337,16 @@ int gctf_synchronize(gctf_ctx* ctx)
 	return GCTF_OK;
 }
 
-int gctf_tune(gctf_ctx* ctx, int chunk, int tile_splits, int use_graph)
+int gctf_tune(gctf_ctx* ctx, int chunk, int tile_splits, int use_graph, int search_block, int list_queue)
 {
 	if (!ctx) return GCTF_ERR_ARG;
+	if (search_block != 0 && search_block != 4 && search_block != 8 && search_block != 16)
+		return fail(ctx, GCTF_ERR_ARG, "search blocking: 4, 8 or 16 candidates per workgroup");
 	if (chunk > 0) ctx->chunk = chunk;
 	if (tile_splits > 0) { ctx->tile_splits = tile_splits; ctx->gen++; }
 	if (use_graph != 0) ctx->use_graph = use_graph > 0 ? 1 : 0;
+	if (search_block != 0) ctx->cpb = search_block;
+	if (list_queue != 0) ctx->list_queue = list_queue > 0 ? 1 : 0;
 	return GCTF_OK;
 }
 
@@ -1549,6 +1595,8 @@ int gctf_do2d_batch_static(gctf_ctx* ctx, const float* d_spect, int n, gctf_resu
 		return fail(ctx, GCTF_ERR_UNSUPPORTED, "correlation ring of " + std::to_string(sp->ring.n)
 			+ " pixels above the static scratch's " + std::to_string(kStaticMaxRing));
 	if (tt.nbins > kStaticMaxBins) return fail(ctx, GCTF_ERR_UNSUPPORTED, "radius bins above the static scratch");
+	if (s1d_part_ld(ctx) > kStaticS1dLd)
+		return fail(ctx, GCTF_ERR_UNSUPPORTED, "configured 1D scan longer than the static scratch's 501 points");
 	char* base = static_cast<char*>(ctx->static_scratch);
 	SearchState* st = reinterpret_cast<SearchState*>(base + kStStates);
 	ImgPlan* plans = reinterpret_cast<ImgPlan*>(base + kStPlans);
@@ -1577,7 +1625,8 @@ int gctf_do2d_batch_static(gctf_ctx* ctx, const float* d_spect, int n, gctf_resu
 	k_state_init<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n, p0, prange);
 	HIPCHK(ctx, hipGetLastError());
 	CHK(fill_plans_at(ctx, plans, n, *sp));
-	CHK(launch_1d_at(ctx, tt, d_spect, radial, plans, st, n, 0, 0.f, 0.f));
+	CHK(launch_1d_at(ctx, tt, d_spect, radial, plans, st, n, 0, 0.f, 0.f, reinterpret_cast<float2*>(base + kStS1d),
+		kStaticS1dLd, s1d_groups(ctx, prange > 0.0f ? 37 : 1)));
 	CHK(run_2d_program(ctx, v, n, NX_BEGIN_DO2D, plain_program(), prange));
 	k_results_out<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n, d_out);
 	HIPCHK(ctx, hipGetLastError());
@@ -1835,7 +1884,7 @@ int record_batch(gctf_ctx* c, const RescalePlan& rp, SearchPlan& sp, TileTables&
 		return GCTF_OK;
 	}
 	CHK(fill_plans(c, 0, n, sp));
-	CHK(launch_1d(c, tt, 0, n, 0, 0.f, 0.f));
+	CHK(launch_1d(c, tt, 0, n, 0, 0.f, 0.f, c->prm.ext_phase_range_deg > 0.0f));
 	CHK(launch_2d(c, sp.T, 0, n, NX_BEGIN_DO2D, std::max(c->prm.ext_phase_range_deg, 0.0f)));
 	return GCTF_OK;
 }
@@ -1853,16 +1902,17 @@ std::string shape_key(int nx, int ny, int dtype, float pix)
 // shape's tuning never reaches a call on another shape.
 struct TuneScope {
 	gctf_ctx* c;
-	int chunk, splits;
+	int chunk, splits, cpb;
 	TuneScope(gctf_ctx* ctx, int nx, int ny, int dtype, float pix)
-		: c(ctx), chunk(ctx->chunk), splits(ctx->tile_splits)
+		: c(ctx), chunk(ctx->chunk), splits(ctx->tile_splits), cpb(ctx->cpb)
 	{
 		auto it = ctx->tuned.find(shape_key(nx, ny, dtype, pix));
 		if (it == ctx->tuned.end()) return;
-		ctx->chunk = it->second.first;
-		ctx->tile_splits = it->second.second;
+		ctx->chunk = it->second.chunk;
+		ctx->tile_splits = it->second.splits;
+		ctx->cpb = it->second.cpb;
 	}
-	~TuneScope() { c->chunk = chunk; c->tile_splits = splits; }
+	~TuneScope() { c->chunk = chunk; c->tile_splits = splits; c->cpb = cpb; }
 	TuneScope(const TuneScope&) = delete;
 	TuneScope& operator=(const TuneScope&) = delete;
 };
@@ -1934,8 +1984,8 @@ int run_batch(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int
 		CHK(ensure_buf(ctx, ctx->props, sizeof(float) * 4 * n));
 	}
 	char key[320];
-	snprintf(key, sizeof key, "%p/%d/%d/%d/%d/%lld/%a/%d/%d/%s", dev_imgs, dtype, n, nx, ny, stride, pix, C,
-		ctx->tile_splits, job_key);
+	snprintf(key, sizeof key, "%p/%d/%d/%d/%d/%lld/%a/%d/%d/%d/%s", dev_imgs, dtype, n, nx, ny, stride, pix, C,
+		ctx->tile_splits, ctx->cpb, job_key);
 	CHK(run_graph(ctx, key, [&]() { return record_batch(ctx, rp, *sp, *tt, dev_imgs, dtype, n, stride, job); }));
 	return read_results(ctx, n, out);
 }
@@ -2082,13 +2132,14 @@ int gctf_estimate_batch_learned(gctf_ctx* ctx, const void* dev_imgs, int dtype, 
 		const float range = std::max(ctx->prm.ext_phase_range_deg, 0.0f);
 		SearchState* st = ctx->states.as<SearchState>();
 		// one graph for every fallback pattern: which micrographs sit out is data
+		TuneScope tune(ctx, nx, ny, dtype, pix);
 		char fkey[160];
-		snprintf(fkey, sizeof fkey, "fallback/%d/%d/%a/%a/%a", n, ctx->prm.tile_size, sp->pix, range,
-			ctx->prm.ext_phase_deg);
+		snprintf(fkey, sizeof fkey, "fallback/%d/%d/%a/%a/%a/%d", n, ctx->prm.tile_size, sp->pix, range,
+			ctx->prm.ext_phase_deg, ctx->cpb);
 		CHK(run_graph(ctx, fkey, [&]() -> int {
 			k_fallback_prepare<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n, ctx->prm.ext_phase_deg, range);
 			CHK(fill_plans(ctx, 0, n, *sp));
-			CHK(launch_1d(ctx, *tt, 0, n, 0, 0.f, 0.f));
+			CHK(launch_1d(ctx, *tt, 0, n, 0, 0.f, 0.f, range > 0.0f));
 			CHK(launch_2d(ctx, ctx->prm.tile_size, 0, n, NX_BEGIN_DO2D, range));
 			k_clear_skip<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n);
 			HIPCHK(ctx, hipGetLastError());
@@ -2245,7 +2296,7 @@ int gctf_estimate_series(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, 
 		CHK(fill_plans(ctx, 0, n, *sp));
 		const int k0 = order[0];
 		// mProcessFull for the first package (:155-166)
-		CHK(launch_1d(ctx, *tt, k0, 1, 0, 0.f, 0.f));
+		CHK(launch_1d(ctx, *tt, k0, 1, 0, 0.f, 0.f, ctx->prm.ext_phase_range_deg > 0.0f));
 		CHK(launch_2d(ctx, T, k0, 1, NX_BEGIN_DO2D, range));
 		k_series_set_ref<<<1, 1, 0, ctx->stream>>>(ctl, k0);
 		for (int j = 1; j < n; ++j) {
@@ -2254,7 +2305,7 @@ int gctf_estimate_series(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, 
 				phase_on);
 			CHK(launch_2d(ctx, T, k, 1, NX_BEGIN_REFINE, refine_phase));
 			k_series_after_refine<<<1, 1, 0, ctx->stream>>>(st, ctl, k, p0, range);
-			CHK(launch_1d(ctx, *tt, k, 1, 0, 0.f, 0.f));
+			CHK(launch_1d(ctx, *tt, k, 1, 0, 0.f, 0.f, ctx->prm.ext_phase_range_deg > 0.0f));
 			CHK(launch_2d(ctx, T, k, 1, NX_BEGIN_DO2D, range));
 			k_series_after_full<<<1, 1, 0, ctx->stream>>>(st, ctl, k);
 		}
@@ -2262,17 +2313,20 @@ int gctf_estimate_series(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, 
 		return GCTF_OK;
 	};
 	char key[192];
-	snprintf(key, sizeof key, "series/%p/%d/%d/%d/%d/%lld/%a/%d/%d/%d/%d", dev_imgs, dtype, n, nx, ny, stride, pix,
-		C, ctx->tile_splits, tomo, z);
+	snprintf(key, sizeof key, "series/%p/%d/%d/%d/%d/%lld/%a/%d/%d/%d/%d/%d", dev_imgs, dtype, n, nx, ny, stride,
+		pix, C, ctx->tile_splits, ctx->cpb, tomo, z);
 	CHK(run_graph(ctx, key, record));
 	return read_results(ctx, n, out);
 }
 
 // ---- uneven batch ------------------------------------------------------------
-// ---- tile work lists (uneven micrographs) -----------------------------------
-// The tile passes of a list run as one launch each over a group of jobs:
-// consecutive micrographs read with the same pixel type whose row-pass
-// output fits kListRtBytes.  Buffer offsets restart at every group.
+// The micrographs of a list run in groups of consecutive jobs read with the
+// same pixel type whose row-pass output fits kListRtBytes; buffer offsets
+// restart at every group.  In a group, each rescale stage is one launch per
+// kernel class over all its micrographs (k_*_list), and the tile passes are
+// one launch each over the group's tile work list: persistent workgroups
+// pulling items from a work queue (k_tile_*_queue), or one workgroup per
+// item (k_tile_*_list) with gctf_tune's list mode 0.
 constexpr size_t kListRtBytes = 8ull << 30;
 
 struct ListGroup {
@@ -2281,23 +2335,138 @@ struct ListGroup {
 	int items = 0;           // row-pass items
 	int max_nr = 0;          // largest block-row count among jobs that need row sums
 	int max_np = 0, max_split = 0;
-	size_t pad = 0, rowsum = 0, sums = 0, rt = 0, part = 0;   // elements
+	size_t pad = 0, rowsum = 0, sums = 0, rt = 0, part = 0, cbuf = 0;   // elements
+	int q = 0;               // first of its two work-queue counters
 };
 
+// One rescale stage's launch over the micrographs of one kernel class.
+enum RsStage { RS_R2C = 0, RS_TRANSPOSE = 1, RS_COLS = 2, RS_C2R = 3 };
+struct RsClass {
+	int stage = 0, variant = 0, maxn = 0, f16 = 0;   // variant: half rows (R2C, C2R) or split columns
+	std::vector<int> entries, first;                 // rescale jobs and their first items
+	int items = 0;
+	size_t first_off = 0, job_off = 0;               // offsets into the uploaded int table
+};
+
+int pow2_class(int m) { return m <= 2048 ? 2048 : m <= 4096 ? 4096 : m <= 8192 ? 8192 : 16384; }
+
+// Class keys and workgroup items of the four stages of one micrograph.
+int rescale_classes(gctf_ctx* c, const RescalePlan& rp, int f16, int key[4][3], int items[4])
+{
+	const int ncx = rp.nnx / 2 + 1;
+	if (rp.rows_half && rp.rows->m > rp.rows_maxn) return fail(c, GCTF_ERR_UNSUPPORTED, "row length");
+	if (rp.cols_split && std::max(rp.cf->m, rp.ci->m) > rp.cols_maxn)
+		return fail(c, GCTF_ERR_UNSUPPORTED, "column length");
+	if (rp.c2r_half && rp.c2r->m > rp.c2r_maxn) return fail(c, GCTF_ERR_UNSUPPORTED, "row length");
+	key[RS_R2C][0] = rp.rows_half; key[RS_R2C][1] = rp.rows_half ? rp.rows_maxn : pow2_class(rp.rows->m);
+	key[RS_R2C][2] = f16;
+	items[RS_R2C] = rp.rows_half ? rp.ny : (rp.ny + 1) / 2;
+	key[RS_TRANSPOSE][0] = 0; key[RS_TRANSPOSE][1] = 0; key[RS_TRANSPOSE][2] = 0;
+	items[RS_TRANSPOSE] = ((ncx + 31) / 32) * ((rp.ny + 31) / 32);
+	key[RS_COLS][0] = rp.cols_split;
+	key[RS_COLS][1] = rp.cols_split ? rp.cols_maxn : pow2_class(std::max(rp.cf->m, rp.ci->m));
+	key[RS_COLS][2] = 0;
+	items[RS_COLS] = 8 * ((ncx + 7) / 8);                // a multiple of 8: xcd_group sees the XCD
+	key[RS_C2R][0] = rp.c2r_half; key[RS_C2R][1] = rp.c2r_half ? rp.c2r_maxn : pow2_class(rp.c2r->m);
+	key[RS_C2R][2] = 0;
+	items[RS_C2R] = rp.c2r_half ? rp.nny : (rp.nny + 1) / 2;
+	return GCTF_OK;
+}
+
+#define RS_LAUNCH(name, kern, NT)                                                                   \
+	PROF(c, name, kern<<<k.items, NT, 0, s>>>(jobs, first, job, n))
+
+template <typename Tin>
+int launch_rs_r2c(gctf_ctx* c, const RsClass& k, const RescaleJob* jobs, const int* first, const int* job)
+{
+	hipStream_t s = c->stream;
+	const int n = (int)k.entries.size();
+	if (k.variant) {
+		if (k.maxn == 8192) RS_LAUNCH("k_rescale_rows_r2c_list", (k_rescale_rows_r2c_list<Tin, 1024, 8192, true>), 1024);
+		else RS_LAUNCH("k_rescale_rows_r2c_list", (k_rescale_rows_r2c_list<Tin, 1024, 16384, true>), 1024);
+	} else if (k.maxn == 2048) RS_LAUNCH("k_rescale_rows_r2c_list", (k_rescale_rows_r2c_list<Tin, 256, 2048, false>), 256);
+	else if (k.maxn == 4096) RS_LAUNCH("k_rescale_rows_r2c_list", (k_rescale_rows_r2c_list<Tin, 512, 4096, false>), 512);
+	else if (k.maxn == 8192) RS_LAUNCH("k_rescale_rows_r2c_list", (k_rescale_rows_r2c_list<Tin, 1024, 8192, false>), 1024);
+	else RS_LAUNCH("k_rescale_rows_r2c_list", (k_rescale_rows_r2c_list<Tin, 1024, 16384, false>), 1024);
+	return GCTF_OK;
+}
+
+int launch_rs_class(gctf_ctx* c, const RsClass& k, const RescaleJob* jobs, const int* tab)
+{
+	hipStream_t s = c->stream;
+	const int* first = tab + k.first_off;
+	const int* job = tab + k.job_off;
+	const int n = (int)k.entries.size();
+	switch (k.stage) {
+	case RS_R2C:
+		CHK(k.f16 ? launch_rs_r2c<_Float16>(c, k, jobs, first, job) : launch_rs_r2c<float>(c, k, jobs, first, job));
+		break;
+	case RS_TRANSPOSE:
+		RS_LAUNCH("k_transpose_list", k_transpose_list, 256);
+		break;
+	case RS_COLS:
+		if (k.variant) {
+			if (k.maxn == 8192) RS_LAUNCH("k_rescale_cols_list", (k_rescale_cols_list<1024, 8192, true>), 1024);
+			else RS_LAUNCH("k_rescale_cols_list", (k_rescale_cols_list<1024, 16384, true>), 1024);
+		} else if (k.maxn == 2048) RS_LAUNCH("k_rescale_cols_list", (k_rescale_cols_list<256, 2048, false>), 256);
+		else if (k.maxn == 4096) RS_LAUNCH("k_rescale_cols_list", (k_rescale_cols_list<512, 4096, false>), 512);
+		else if (k.maxn == 8192) RS_LAUNCH("k_rescale_cols_list", (k_rescale_cols_list<1024, 8192, false>), 1024);
+		else RS_LAUNCH("k_rescale_cols_list", (k_rescale_cols_list<1024, 16384, false>), 1024);
+		break;
+	default:
+		if (k.variant) {
+			if (k.maxn == 8192) RS_LAUNCH("k_rescale_rows_c2r_list", (k_rescale_rows_c2r_list<1024, 8192, true>), 1024);
+			else RS_LAUNCH("k_rescale_rows_c2r_list", (k_rescale_rows_c2r_list<1024, 16384, true>), 1024);
+		} else if (k.maxn == 2048) RS_LAUNCH("k_rescale_rows_c2r_list", (k_rescale_rows_c2r_list<256, 2048, false>), 256);
+		else if (k.maxn == 4096) RS_LAUNCH("k_rescale_rows_c2r_list", (k_rescale_rows_c2r_list<512, 4096, false>), 512);
+		else if (k.maxn == 8192) RS_LAUNCH("k_rescale_rows_c2r_list", (k_rescale_rows_c2r_list<1024, 8192, false>), 1024);
+		else RS_LAUNCH("k_rescale_rows_c2r_list", (k_rescale_rows_c2r_list<1024, 16384, false>), 1024);
+		break;
+	}
+	HIPCHK(c, hipGetLastError());
+	return GCTF_OK;
+}
+#undef RS_LAUNCH
+
+// Workgroups of a persistent launch: as many as stay resident on the device
+// at once (occupancy x compute units), at most `items`.
+template <class K>
+int resident_grid(gctf_ctx* c, K kernel, int threads, int items, int* out)
+{
+	int per_cu = 0, cus = 0;
+	HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+	HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+	*out = std::max(1, std::min(items, std::max(1, per_cu) * std::max(1, cus)));
+	return GCTF_OK;
+}
+
 template <typename Tin, int T>
-int launch_tile_list_t(gctf_ctx* c, const TileTables& tt, const TileJob* d_jobs, const ListGroup& G)
+int launch_tile_list_t(gctf_ctx* c, const TileTables& tt, const TileJob* d_jobs, const ListGroup& G, int* qnext)
 {
 	hipStream_t s = c->stream;
 	constexpr int CX = T / 2 + 1, ROWS = 16, RG = 2, CPW = 4;
+	constexpr int NCG = (CX + CPW - 1) / CPW;
 	const TileJob* jobs = d_jobs + G.j0;
 	if (G.max_nr > 0)
 		PROF(c, "k_row_sums", k_row_sums_list<Tin><<<dim3(G.max_nr, G.nj), 256, 0, s>>>(jobs, c->rowsum.as<float>()));
 	PROF(c, "k_block_rows", k_block_rows_list<<<dim3((G.max_np + 255) / 256, G.nj), 256, 0, s>>>(jobs,
 		c->rowsum.as<float>(), c->bsums.as<double>()));
-	PROF(c, "k_tile_rows", k_tile_rows_list<Tin, T, ROWS, RG><<<G.items, (ROWS / 2) * (T / 8), 0, s>>>(jobs, G.nj,
-		c->bsums.as<double>(), tt.mask.as<float>(), tt.tw.as<float2>(), c->rt.as<float2>()));
-	PROF(c, "k_tile_cols", k_tile_cols_list<T, CPW><<<dim3((CX + CPW - 1) / CPW, G.max_split, G.nj), CPW * (T / 8),
-		0, s>>>(jobs, c->rt.as<float2>(), tt.tw.as<float2>(), c->tpart.as<float>()));
+	const int col_items = G.nj * G.max_split * NCG;
+	if (c->list_queue) {
+		int gr = 0, gc = 0;
+		CHK(resident_grid(c, k_tile_rows_queue<Tin, T, ROWS, RG>, (ROWS / 2) * (T / 8), G.items, &gr));
+		CHK(resident_grid(c, k_tile_cols_queue<T, CPW>, CPW * (T / 8), col_items, &gc));
+		PROF(c, "k_tile_rows_queue", k_tile_rows_queue<Tin, T, ROWS, RG><<<gr, (ROWS / 2) * (T / 8), 0, s>>>(jobs,
+			G.nj, G.items, qnext + G.q, c->bsums.as<double>(), tt.mask.as<float>(), tt.tw.as<float2>(),
+			c->rt.as<float2>()));
+		PROF(c, "k_tile_cols_queue", k_tile_cols_queue<T, CPW><<<gc, CPW * (T / 8), 0, s>>>(jobs, G.nj, G.max_split,
+			qnext + G.q + 1, c->rt.as<float2>(), tt.tw.as<float2>(), c->tpart.as<float>()));
+	} else {
+		PROF(c, "k_tile_rows", k_tile_rows_list<Tin, T, ROWS, RG><<<G.items, (ROWS / 2) * (T / 8), 0, s>>>(jobs, G.nj,
+			c->bsums.as<double>(), tt.mask.as<float>(), tt.tw.as<float2>(), c->rt.as<float2>()));
+		PROF(c, "k_tile_cols", k_tile_cols_list<T, CPW><<<dim3(NCG, G.max_split, G.nj), CPW * (T / 8),
+			0, s>>>(jobs, c->rt.as<float2>(), tt.tw.as<float2>(), c->tpart.as<float>()));
+	}
 	PROF(c, "k_tile_reduce", k_tile_reduce_list<<<dim3((T * CX + 255) / 256, G.nj), 256, 0, s>>>(jobs,
 		c->tpart.as<float>(), T * CX, c->raw.as<float>()));
 	HIPCHK(c, hipGetLastError());
@@ -2305,19 +2474,27 @@ int launch_tile_list_t(gctf_ctx* c, const TileTables& tt, const TileJob* d_jobs,
 }
 
 template <int T>
-int launch_tile_list_tt(gctf_ctx* c, const TileTables& tt, const TileJob* d_jobs, const ListGroup& G)
+int launch_tile_list_tt(gctf_ctx* c, const TileTables& tt, const TileJob* d_jobs, const ListGroup& G, int* qnext)
 {
-	return G.f16 ? launch_tile_list_t<_Float16, T>(c, tt, d_jobs, G) : launch_tile_list_t<float, T>(c, tt, d_jobs, G);
+	return G.f16 ? launch_tile_list_t<_Float16, T>(c, tt, d_jobs, G, qnext)
+		: launch_tile_list_t<float, T>(c, tt, d_jobs, G, qnext);
 }
 
-int launch_tile_list(gctf_ctx* c, int T, const TileTables& tt, const TileJob* d_jobs, const ListGroup& G)
+int launch_tile_list(gctf_ctx* c, int T, const TileTables& tt, const TileJob* d_jobs, const ListGroup& G, int* qnext)
 {
 	switch (T) {
-	case 256: return launch_tile_list_tt<256>(c, tt, d_jobs, G);
-	case 512: return launch_tile_list_tt<512>(c, tt, d_jobs, G);
-	case 1024: return launch_tile_list_tt<1024>(c, tt, d_jobs, G);
+	case 256: return launch_tile_list_tt<256>(c, tt, d_jobs, G, qnext);
+	case 512: return launch_tile_list_tt<512>(c, tt, d_jobs, G, qnext);
+	case 1024: return launch_tile_list_tt<1024>(c, tt, d_jobs, G, qnext);
 	default: return fail(c, GCTF_ERR_UNSUPPORTED, "tile size");
 	}
+}
+
+// Per-micrograph plans of a list, copied from their uploaded table.
+__global__ void k_copy_plans(ImgPlan* __restrict__ dst, const ImgPlan* __restrict__ src, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) dst[i] = src[i];
 }
 
 int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result* out)
@@ -2330,6 +2507,7 @@ int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result
 		RescalePlan rp;
 		TileGrid g;
 		SearchPlan* sp;
+		int splits;
 	};
 	std::vector<Item> it((size_t)n);
 	TileTables* tt;
@@ -2347,15 +2525,18 @@ int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result
 		if (it[i].g.ntx < 1 || it[i].g.nty < 1) return fail(ctx, GCTF_ERR_ARG, "no tile fits the image");
 		CHK(get_search_plan(ctx, ctx->prm.kv, ctx->prm.cs_mm, ctx->prm.amp_contrast, rp.pixn, T, &it[i].sp));
 		kmax = std::max(kmax, it[i].sp->ring.n);
+		// tile splits: gctf_autotune's for this shape, else the context's
+		auto tu = ctx->tuned.find(shape_key(im.nx, im.ny, im.dtype, im.pix));
+		it[i].splits = tu == ctx->tuned.end() ? ctx->tile_splits : tu->second.splits;
 		char k[128];
-		snprintf(k, sizeof k, "/%p:%d:%d:%d:%a", im.data, im.dtype, im.nx, im.ny, im.pix);
+		snprintf(k, sizeof k, "/%p:%d:%d:%d:%a:%d", im.data, im.dtype, im.nx, im.ny, im.pix, it[i].splits);
 		key += k;
 	}
 	CHK(ensure_image_buffers(ctx, n, T, kmax));
 	ctx->kstride = kmax;
 	// tile work list: micrographs read as float32 (rescaled ones and float32
 	// inputs) first, then those read as float16; groups of consecutive jobs
-	const int CX = T / 2 + 1, NRB = T / 16;
+	const int NRB = T / 16;
 	std::vector<int> order;
 	for (int pass = 0; pass < 2; ++pass)
 		for (int i = 0; i < n; ++i) {
@@ -2364,34 +2545,40 @@ int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result
 		}
 	std::vector<TileJob> jobs((size_t)n);
 	std::vector<ListGroup> groups;
-	std::vector<size_t> pad_off((size_t)n, 0);
+	std::vector<size_t> pad_off((size_t)n, 0), cbuf_off((size_t)n, 0);
 	for (int k = 0; k < n; ++k) {
 		const int i = order[k];
 		const TileGrid& g = it[i].g;
-		const int f16 = !it[i].rp.active && imgs[i].dtype == GCTF_DTYPE_F16;
+		const RescalePlan& rp = it[i].rp;
+		const int f16 = !rp.active && imgs[i].dtype == GCTF_DTYPE_F16;
 		const int ntiles = g.ntx * g.nty;
-		const size_t rt = (size_t)ntiles * CX * T;
+		const size_t rt = (size_t)ntiles * (T / 2 + 1) * T;
 		if (groups.empty() || groups.back().f16 != f16
 			|| (groups.back().rt + rt) * sizeof(float2) > kListRtBytes) {
 			ListGroup G;
 			G.j0 = k; G.f16 = f16;
+			G.q = 2 * (int)groups.size();
 			groups.push_back(G);
 		}
 		ListGroup& G = groups.back();
 		TileJob& J = jobs[(size_t)k];
 		J.img = nullptr;
-		J.pitch = it[i].rp.active ? it[i].rp.pitch : imgs[i].nx;
+		J.pitch = rp.active ? rp.pitch : imgs[i].nx;
 		J.g = g;
 		J.rows_first = G.items;
-		J.nsplit = std::min(ctx->tile_splits, ntiles);
-		J.need_rows = it[i].rp.active ? 0 : 1;
+		J.nsplit = std::min(it[i].splits, ntiles);
+		J.need_rows = rp.active ? 0 : 1;
 		J.raw_slot = i;
 		const int nr = (g.nby - 1) * g.step + g.bsize;
 		J.rowsum_off = (long long)G.rowsum; G.rowsum += (size_t)nr * g.nbx;
 		J.sums_off = (long long)G.sums; G.sums += (size_t)g.nbx * g.nby * kSumParts;
 		J.rt_off = (long long)G.rt; G.rt += rt;
-		J.part_off = (long long)G.part; G.part += (size_t)J.nsplit * T * CX;
-		if (it[i].rp.active) { pad_off[(size_t)i] = G.pad; G.pad += (size_t)it[i].rp.pitch * it[i].rp.nny; }
+		J.part_off = (long long)G.part; G.part += (size_t)J.nsplit * T * (T / 2 + 1);
+		if (rp.active) {
+			pad_off[(size_t)i] = G.pad; G.pad += (size_t)rp.pitch * rp.nny;
+			const size_t ncx = rp.nnx / 2 + 1;
+			cbuf_off[(size_t)i] = G.cbuf; G.cbuf += std::max((size_t)rp.ny * ncx, ncx * (size_t)rp.nny);
+		}
 		G.items += (ntiles * NRB + 1) / 2;
 		if (J.need_rows) G.max_nr = std::max(G.max_nr, nr);
 		G.max_np = std::max(G.max_np, g.nbx * g.nby * kSumParts);
@@ -2402,47 +2589,110 @@ int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result
 	for (const ListGroup& G : groups) {
 		most.pad = std::max(most.pad, G.pad); most.rowsum = std::max(most.rowsum, G.rowsum);
 		most.sums = std::max(most.sums, G.sums); most.rt = std::max(most.rt, G.rt);
-		most.part = std::max(most.part, G.part);
+		most.part = std::max(most.part, G.part); most.cbuf = std::max(most.cbuf, G.cbuf);
 	}
-	for (int i = 0; i < n; ++i)
-		if (it[i].rp.active) CHK(ensure_rescale_buffers(ctx, it[i].rp, 1));
+	CHK(ensure_buf(ctx, ctx->cbufA, sizeof(float2) * std::max<size_t>(most.cbuf, 1)));
+	CHK(ensure_buf(ctx, ctx->cbufB, sizeof(float2) * std::max<size_t>(most.cbuf, 1)));
 	CHK(ensure_buf(ctx, ctx->padc, sizeof(float) * std::max<size_t>(most.pad, 1)));
 	CHK(ensure_buf(ctx, ctx->rowsum, sizeof(float) * most.rowsum));
 	CHK(ensure_buf(ctx, ctx->bsums, sizeof(double) * most.sums));
 	CHK(ensure_buf(ctx, ctx->rt, sizeof(float2) * most.rt));
 	CHK(ensure_buf(ctx, ctx->tpart, sizeof(float) * most.part));
 	CHK(ensure_buf(ctx, ctx->jobs, sizeof(TileJob) * (size_t)n));
+	CHK(ensure_buf(ctx, ctx->qnext, sizeof(int) * 2 * groups.size()));
 	for (int k = 0; k < n; ++k) {
 		const int i = order[k];
 		jobs[(size_t)k].img = it[i].rp.active ? (const void*)(ctx->padc.as<float>() + pad_off[(size_t)i])
 			: imgs[i].data;
 	}
+	// rescale jobs and their kernel classes, per group in stage order
+	std::vector<RescaleJob> rjobs;
+	std::vector<std::vector<RsClass>> gclasses(groups.size());
+	std::vector<int> tab;
+	for (size_t gi = 0; gi < groups.size(); ++gi) {
+		const ListGroup& G = groups[gi];
+		std::map<std::vector<int>, size_t> index;   // (stage, variant, maxn, f16) -> class
+		auto& cls = gclasses[gi];
+		for (int k = G.j0; k < G.j0 + G.nj; ++k) {
+			const int i = order[k];
+			const RescalePlan& rp = it[i].rp;
+			if (!rp.active) continue;
+			RescaleJob R;
+			R.img = imgs[i].data;
+			R.nx = rp.nx; R.ny = rp.ny; R.nnx = rp.nnx; R.nny = rp.nny; R.ncx = rp.nnx / 2 + 1;
+			R.pitch = rp.pitch;
+			R.rows = rp.rows->plan(); R.cf = rp.cf->plan(); R.ci = rp.ci->plan(); R.c2r = rp.c2r->plan();
+			R.A = ctx->cbufA.as<float2>() + cbuf_off[(size_t)i];
+			R.B = ctx->cbufB.as<float2>() + cbuf_off[(size_t)i];
+			R.out = ctx->padc.as<float>() + pad_off[(size_t)i];
+			R.rowsum = ctx->rowsum.as<float>() + jobs[(size_t)k].rowsum_off;
+			R.g = it[i].g;
+			const int rj = (int)rjobs.size();
+			rjobs.push_back(R);
+			int ck[4][3], items[4];
+			CHK(rescale_classes(ctx, rp, imgs[i].dtype == GCTF_DTYPE_F16, ck, items));
+			for (int st = 0; st < 4; ++st) {
+				const std::vector<int> kk = {st, ck[st][0], ck[st][1], ck[st][2]};
+				auto f = index.find(kk);
+				if (f == index.end()) {
+					RsClass C;
+					C.stage = st; C.variant = ck[st][0]; C.maxn = ck[st][1]; C.f16 = ck[st][2];
+					index[kk] = cls.size();
+					cls.push_back(C);
+					f = index.find(kk);
+				}
+				RsClass& C = cls[f->second];
+				C.entries.push_back(rj);
+				C.first.push_back(C.items);
+				C.items += items[st];
+			}
+		}
+		std::stable_sort(cls.begin(), cls.end(), [](const RsClass& a, const RsClass& b) { return a.stage < b.stage; });
+		for (RsClass& C : cls) {
+			C.first_off = tab.size();
+			tab.insert(tab.end(), C.first.begin(), C.first.end());
+			C.job_off = tab.size();
+			tab.insert(tab.end(), C.entries.begin(), C.entries.end());
+		}
+	}
+	// per-micrograph search plans, copied into place by one launch in the graph
+	std::vector<ImgPlan> hplans((size_t)n);
+	for (int i = 0; i < n; ++i) {
+		const SearchPlan& sp = *it[i].sp;
+		hplans[i].cc = sp.cc; hplans[i].ring = sp.ring; hplans[i].lo1d = sp.lo1d; hplans[i].hi1d = sp.hi1d;
+	}
+	CHK(ensure_buf(ctx, ctx->rjobs, sizeof(RescaleJob) * std::max<size_t>(rjobs.size(), 1)));
+	CHK(ensure_buf(ctx, ctx->rtab, sizeof(int) * std::max<size_t>(tab.size(), 1)));
+	CHK(ensure_buf(ctx, ctx->list_plans, sizeof(ImgPlan) * (size_t)n));
 	// ordered on the context's stream (it may be a non-blocking one)
 	HIPCHK(ctx, hipMemcpyAsync(ctx->jobs.p, jobs.data(), sizeof(TileJob) * (size_t)n, hipMemcpyHostToDevice,
 		ctx->stream));
+	if (!rjobs.empty())
+		HIPCHK(ctx, hipMemcpyAsync(ctx->rjobs.p, rjobs.data(), sizeof(RescaleJob) * rjobs.size(), hipMemcpyHostToDevice,
+			ctx->stream));
+	if (!tab.empty())
+		HIPCHK(ctx, hipMemcpyAsync(ctx->rtab.p, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice,
+			ctx->stream));
+	HIPCHK(ctx, hipMemcpyAsync(ctx->list_plans.p, hplans.data(), sizeof(ImgPlan) * (size_t)n, hipMemcpyHostToDevice,
+		ctx->stream));
 	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
 	char tail[64];
-	snprintf(tail, sizeof tail, "/%d", ctx->tile_splits);
+	snprintf(tail, sizeof tail, "/%d/%d/%d", ctx->tile_splits, ctx->cpb, ctx->list_queue);
 	key += tail;
 	auto record = [&]() -> int {
+		const int nq = 2 * (int)groups.size();
+		k_queue_reset<<<(nq + 63) / 64, 64, 0, ctx->stream>>>(ctx->qnext.as<int>(), nq);
 		k_state_init<<<(n + 63) / 64, 64, 0, ctx->stream>>>(ctx->states.as<SearchState>(), n,
 			ctx->prm.ext_phase_deg, std::max(ctx->prm.ext_phase_range_deg, 0.0f));
-		for (const ListGroup& G : groups) {
-			for (int k = G.j0; k < G.j0 + G.nj; ++k) {
-				const int i = order[k];
-				const gctf_image& im = imgs[i];
-				const RescalePlan& rp = it[i].rp;
-				if (!rp.active) continue;
-				const long long st = (long long)im.nx * im.ny;
-				const long long ps = (long long)rp.pitch * rp.nny;
-				CHK(launch_rescale(ctx, rp, im.data, im.dtype, st, 1, ctx->padc.as<float>() + pad_off[(size_t)i], ps,
-					&it[i].g, ctx->rowsum.as<float>() + jobs[(size_t)k].rowsum_off));
-			}
-			CHK(launch_tile_list(ctx, T, *tt, ctx->jobs.as<TileJob>(), G));
+		for (size_t gi = 0; gi < groups.size(); ++gi) {
+			for (const RsClass& C : gclasses[gi])
+				CHK(launch_rs_class(ctx, C, ctx->rjobs.as<RescaleJob>(), ctx->rtab.as<int>()));
+			CHK(launch_tile_list(ctx, T, *tt, ctx->jobs.as<TileJob>(), groups[gi], ctx->qnext.as<int>()));
 		}
 		CHK(launch_post(ctx, *tt, n));
-		for (int i = 0; i < n; ++i) CHK(fill_plans(ctx, i, 1, *it[i].sp));
-		CHK(launch_1d(ctx, *tt, 0, n, 0, 0.f, 0.f));
+		k_copy_plans<<<(n + 63) / 64, 64, 0, ctx->stream>>>(ctx->plans.as<ImgPlan>(), ctx->list_plans.as<ImgPlan>(), n);
+		HIPCHK(ctx, hipGetLastError());
+		CHK(launch_1d(ctx, *tt, 0, n, 0, 0.f, 0.f, ctx->prm.ext_phase_range_deg > 0.0f));
 		CHK(launch_2d(ctx, T, 0, n, NX_BEGIN_DO2D, std::max(ctx->prm.ext_phase_range_deg, 0.0f)));
 		return GCTF_OK;
 	};
@@ -2452,7 +2702,7 @@ int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result
 
 // ---- autotuner ---------------------------------------------------------------
 int gctf_autotune(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny, long long stride,
-	float pix, int* best_chunk, int* best_splits, float* best_ms)
+	float pix, int* best_chunk, int* best_splits, float* best_ms, int* best_search_block)
 {
 	if (!ctx) return GCTF_ERR_ARG;
 	if (!dev_imgs || n < 1) return fail(ctx, GCTF_ERR_ARG, "null input or n < 1");
@@ -2502,14 +2752,51 @@ int gctf_autotune(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx,
 			if (tmin < best) { best = tmin; bc = ci; bs = si; }
 		}
 	}
+	// search blocking: the plain 1D + 2D search of this batch's spectra with
+	// 4, 8 and 16 candidates per evaluator workgroup
+	const int save_b = ctx->cpb;
+	int bb = save_b;
+	if (rc == GCTF_OK) {
+		ctx->chunk = bc;
+		ctx->tile_splits = bs;
+		rc = ensure_spectrum_buffers(ctx, g, T, std::min(bc, n));
+		if (rc == GCTF_OK && rp.active) rc = ensure_rescale_buffers(ctx, rp, std::min(bc, n));
+		if (rc == GCTF_OK && rp.active)
+			rc = ensure_buf(ctx, ctx->padc, sizeof(float) * (size_t)rp.pitch * rp.nny * std::min(bc, n));
+		ctx->kstride = sp->ring.n;
+		if (rc == GCTF_OK) rc = record_spectra(ctx, rp, *tt, dev_imgs, dtype, n, stride);
+		float sbest = 1e30f;
+		const float p0 = ctx->prm.ext_phase_deg, prange = std::max(ctx->prm.ext_phase_range_deg, 0.0f);
+		for (int cb : {4, 8, 16}) {
+			if (rc != GCTF_OK) break;
+			ctx->cpb = cb;
+			float tmin = 1e30f;
+			for (int rep = 0; rep < 3 && rc == GCTF_OK; ++rep) {
+				(void)hipEventRecord(a, ctx->stream);
+				k_state_init<<<(n + 63) / 64, 64, 0, ctx->stream>>>(ctx->states.as<SearchState>(), n, p0, prange);
+				rc = fill_plans(ctx, 0, n, *sp);
+				if (rc == GCTF_OK) rc = launch_1d(ctx, *tt, 0, n, 0, 0.f, 0.f, prange > 0.0f);
+				if (rc == GCTF_OK) rc = launch_2d(ctx, T, 0, n, NX_BEGIN_DO2D, prange);
+				(void)hipEventRecord(b, ctx->stream);
+				if (hipEventSynchronize(b) != hipSuccess) rc = fail(ctx, GCTF_ERR_HIP, "autotune timing");
+				float ms = 0.f;
+				(void)hipEventElapsedTime(&ms, a, b);
+				if (rep > 0) tmin = std::min(tmin, ms);
+			}
+			if (tmin < sbest) { sbest = tmin; bb = cb; }
+		}
+	}
 	(void)hipEventDestroy(a);
 	(void)hipEventDestroy(b);
 	ctx->chunk = save_c;
 	ctx->tile_splits = save_s;
+	ctx->cpb = save_b;
 	if (rc != GCTF_OK) return rc;
-	ctx->tuned[shape_key(nx, ny, dtype, pix)] = std::make_pair(bc, bs);
+	ctx->tuned[shape_key(nx, ny, dtype, pix)] = gctf_ctx::Tuned{bc, bs, bb};
+	ctx->tune_gen++;
 	if (best_chunk) *best_chunk = bc;
 	if (best_splits) *best_splits = bs;
 	if (best_ms) *best_ms = best;
+	if (best_search_block) *best_search_block = bb;
 	return GCTF_OK;
 }

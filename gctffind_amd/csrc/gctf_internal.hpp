@@ -80,7 +80,12 @@ struct SearchState {
 	int skip;                // 1: the image sits out the running search pass
 	int fallback;            // learned path: 1 when the plain search must decide
 	float conf;              // learned path: proposal confidence
-	int pad_;
+	// pending 1D stage (CFindDefocus1D::mBrutalForceSearch): candidate
+	// i = iph * s1_ndf + idf at defocus s1_df0 + idf s1_dfstep A and phase
+	// s1_ph0 + iph s1_phstep degrees (clamped to [0, 150]); s1_stage 0 is
+	// CFindCtf1D::mFindDefocus, 1 mRefineDefocus
+	float s1_df0, s1_dfstep, s1_ph0, s1_phstep;
+	int s1_ndf, s1_n, s1_stage;
 };
 
 // Per-plan 2D-ring tables in structure-of-arrays form (device pointers).

@@ -36,12 +36,13 @@ __device__ __forceinline__ float sinpi_dev(float v)
 // ---------------------------------------------------------------------------
 // 1D search (CFindDefocus1D::mBrutalForceSearch, CFindDefocus1D.cpp:76-119;
 // CFindCtf1D::mFindDefocus / mRefineDefocus, CFindCtf1D.cpp:73-111).
-// One workgroup per micrograph.
+// Each brute-force stage is one k_s1d_eval launch over all candidates of all
+// micrographs (a wavefront per candidate group, lanes over the radial bins),
+// then one k_s1d_pick per micrograph takes the first strict maximum and sets
+// up the next stage.
 // ---------------------------------------------------------------------------
 struct Search1DParams {
-	CtfConsts c;
 	int cmpx;
-	float lo, hi;            // bins x with lo <= x < hi (Fourier pixels)
 	int mode;                // 0: Do1D, 1: Refine1D(init_df, range)
 	float init_df, range;    // Refine1D arguments
 	// initial Do1D scan (mFindDefocus): defocus [scan_lo, scan_hi] A in
@@ -50,132 +51,206 @@ struct Search1DParams {
 	float scan_lo, scan_hi, scan_step;
 };
 
-__device__ __forceinline__ float cc1d_eval(const Search1DParams& p, const float* rad,
-	const double* w1d, float df_a, float ph_deg)
-{
-	float ext = ph_deg * kD2R1D;                    // CFindDefocus1D.cpp:124
-	float dfp = df_a / p.c.pix;                     // :125
-	float add = p.c.amp_phase + ext;                // GCalcCTF1D.cu:63
-	const double wl = p.c.wl, cs = p.c.cs;
-	double s = 0.0, s1 = 0.0, s2 = 0.0;
-	for (int x = 0; x < p.cmpx; ++x) {
-		if (!((float)x >= p.lo && (float)x < p.hi)) continue;
-		double f2 = (double)x * 0.5 / (double)(p.cmpx - 1);
-		f2 *= f2;
-		double chi = (double)add + (double)kPi1D * wl * f2 * ((double)dfp - 0.5 * wl * wl * f2 * cs);
-		double c = (fabs(sin(chi)) - 0.5) * w1d[x];
-		double r = (double)rad[x];
-		s += c * r; s1 += c * c; s2 += r * r;
-	}
-	if (s1 > 0 && s2 > 0) return (float)(s / sqrt(s1 * s2));
-	return 0.0f;
-}
+// Candidates per wavefront of k_s1d_eval; 4 wavefronts per workgroup.
+constexpr int kS1dCand = 4;
+constexpr int kS1dPerWg = 4 * kS1dCand;
 
-// Brute force over (df, phase); returns the first strict maximum.
-// step > 0 replaces the reference's step (range / 500, at least 10 A); the
-// point count follows its rule either way.
-__device__ void search1d_brute(const Search1DParams& p, const float* rad, const double* w1d,
-	float dfr0, float dfr1, float phr0, float phr1, float step, float* out3, float* s_cc, int* s_i)
+// The brute-force lattice of CFindDefocus1D::mBrutalForceSearch
+// (CFindDefocus1D.cpp:76-96): defocus [dfr0, dfr1] in 500 steps of at least
+// 10 A (or `step` when > 0, the configured scan), phase [phr0, phr1] in 36
+// steps; the point counts follow (int)(range / step) / 2 * 2 + 1.
+__device__ __forceinline__ void s1d_lattice(SearchState& S, float dfr0, float dfr1, float phr0, float phr1,
+	float step)
 {
 	int ndf = 501;
-	float dfrange = dfr1 - dfr0;
+	const float dfrange = dfr1 - dfr0;
 	float dfstep = dfrange / (float)(ndf - 1);
 	if (dfstep < 10) dfstep = 10.0f;
 	if (step > 0.0f) dfstep = step;
 	ndf = (int)(dfrange / dfstep) / 2 * 2 + 1;
 	int nps = 37;
-	float psrange = phr1 - phr0;
-	float psstep = psrange / (float)(nps - 1);
+	const float psrange = phr1 - phr0;
+	const float psstep = psrange / (float)(nps - 1);
 	if (psstep == 0) nps = 1;
 	else nps = (int)(psrange / psstep) / 2 * 2 + 1;
-	const int npts = ndf * nps;
+	S.s1_df0 = dfr0; S.s1_dfstep = dfstep; S.s1_ndf = ndf;
+	S.s1_ph0 = phr0; S.s1_phstep = psstep;
+	S.s1_n = ndf * nps;
+}
+
+// Candidate i's defocus (A) and phase (degrees, clamped to [0, 150]).
+__device__ __forceinline__ void s1d_candidate(const SearchState& S, int i, float& df, float& ph)
+{
+	const int ifo = i % S.s1_ndf, iph = i / S.s1_ndf;
+	df = S.s1_df0 + (float)ifo * S.s1_dfstep;
+	ph = S.s1_ph0 + (float)iph * S.s1_phstep;
+	if (ph < 0) ph = 0.0f;
+	else if (ph > 150.0f) ph = 150.0f;
+}
+
+// CFindCtf1D::mRefineDefocus (CFindCtf1D.cpp:93-111) around df_min with the
+// phase window built from the zero-initialised slot (CFindCtf1D.cpp:104).
+__device__ __forceinline__ void s1d_refine_lattice(SearchState& S, float df_min, float ext_phase, float range,
+	float pix)
+{
+	const float mindf = 3000.0f * pix * pix;
+	const float dfr0 = fmaxf(df_min - range / 2, mindf);
+	const float dfr1 = dfr0 + range;
+	const float prange = 0.2f * S.phase_range;
+	const float phr0 = fmaxf(ext_phase - prange / 2, 0.0f);
+	const float phr1 = fminf(0.0f + prange, 150.0f);
+	s1d_lattice(S, dfr0, dfr1, phr0, phr1, 0.0f);
+	S.s1_stage = 1;
+}
+
+// First stage of Do1D (mFindDefocus) or Refine1D (mRefineDefocus directly).
+__global__ void k_s1d_begin(Search1DParams p, const ImgPlan* __restrict__ plans, SearchState* __restrict__ st,
+	int n)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= n) return;
+	SearchState& S = st[b];
+	if (S.skip) return;
+	const float pix = plans[b].cc.pix;
+	if (p.mode == 0) {
+		const float p2 = pix * pix;
+		const float dfr0 = p.scan_lo > 0.0f ? p.scan_lo : 3000.0f * p2;
+		const float dfr1 = p.scan_hi > 0.0f ? p.scan_hi : 30000.0f * p2;
+		const float phr0 = fmaxf(S.ext_phase - S.phase_range / 2, 0.0f);
+		const float phr1 = fminf(phr0 + S.phase_range, 150.0f);
+		s1d_lattice(S, dfr0, dfr1, phr0, phr1, p.scan_step);
+		S.s1_stage = 0;
+	} else {
+		s1d_refine_lattice(S, p.init_df, S.ext_phase, p.range, pix);
+	}
+}
+
+// CC of the pending stage's candidates (GCalcCTF1D.cu:23-32, GCC1D.cu:15-125):
+// per radial bin x in [lo, hi), c = (|sin(chi)| - 1/2) exp(-(x/256)^2) with
+// chi / pi = ph0 + a_x df - b_x in IEEE single (a_x = (pi'/pi) wl f^2,
+// b_x = (pi'/pi) wl^3 f^4 Cs / 2, pi' the kernel's 3.141592654f), the sums
+// c S and c^2 in FP32 per lane and FP64 across lanes, CC = sum cS /
+// sqrt(sum c^2 sum S^2) (0 when a sum is <= 0).  Each workgroup keeps the
+// first strict maximum of its kS1dPerWg consecutive candidates:
+// part[image][workgroup] = (CC, candidate index as int bits).  Grid
+// (ceil(max candidates / kS1dPerWg), images), 256 threads.
+__global__ __launch_bounds__(256) void k_s1d_eval(int cmpx, const ImgPlan* __restrict__ plans,
+	const float* __restrict__ radial, const double* __restrict__ w1d, const SearchState* __restrict__ st,
+	int ldp, float2* __restrict__ part)
+{
+	__shared__ float s_cc[kS1dPerWg];
+	const SearchState& S = st[blockIdx.y];
+	if (S.skip || S.s1_stage > 1) return;
+	const int n = S.s1_n;
+	if ((int)blockIdx.x * kS1dPerWg >= n) return;
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	const int c0 = blockIdx.x * kS1dPerWg + wave * kS1dCand;
+	const ImgPlan P = plans[blockIdx.y];
+	const float* rad = radial + (long long)blockIdx.y * cmpx;
+	const double wl = P.cc.wl, cs = P.cc.cs;
+	const double pir = (double)kPi1D / 3.14159265358979323846;
+	float dfp[kS1dCand], ph0h[kS1dCand];
+#pragma unroll
+	for (int j = 0; j < kS1dCand; ++j) {
+		float df, ph;
+		s1d_candidate(S, min(c0 + j, n - 1), df, ph);
+		dfp[j] = df / P.cc.pix;                                    // CFindDefocus1D.cpp:125
+		const float add = P.cc.amp_phase + ph * kD2R1D;             // :124, GCalcCTF1D.cu:63
+		ph0h[j] = (float)((double)add / 3.14159265358979323846 - 0.5);
+	}
+	float acs[kS1dCand], acc[kS1dCand], ass = 0.f;
+#pragma unroll
+	for (int j = 0; j < kS1dCand; ++j) { acs[j] = 0.f; acc[j] = 0.f; }
+	for (int x = lane; x < cmpx; x += 64) {
+		if (!((float)x >= P.lo1d && (float)x < P.hi1d)) continue;
+		double f2 = (double)x * 0.5 / (double)(cmpx - 1);
+		f2 *= f2;
+		const float a = (float)(pir * wl * f2);
+		const float b = (float)(pir * 0.5 * wl * wl * wl * f2 * f2 * cs);
+		const float w = (float)w1d[x];
+		const float sv = rad[x];
+#pragma unroll
+		for (int j = 0; j < kS1dCand; ++j) {
+			const float cv = sinpi_dev(fmaf(a, dfp[j], ph0h[j] - b)) * w;
+			acs[j] = fmaf(cv, sv, acs[j]);
+			acc[j] = fmaf(cv, cv, acc[j]);
+		}
+		ass = fmaf(sv, sv, ass);
+	}
+	double dss = ass;
+	for (int o = 32; o > 0; o >>= 1) dss += __shfl_xor(dss, o);
+#pragma unroll
+	for (int j = 0; j < kS1dCand; ++j) {
+		double dcs = acs[j], dcc = acc[j];
+		for (int o = 32; o > 0; o >>= 1) {
+			dcs += __shfl_xor(dcs, o);
+			dcc += __shfl_xor(dcc, o);
+		}
+		if (lane == 0) s_cc[wave * kS1dCand + j] = (dcc > 0 && dss > 0) ? (float)(dcs / sqrt(dcc * dss)) : 0.0f;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		float best = (float)-1e20;
+		int bi = -1;
+		for (int k = 0; k < kS1dPerWg; ++k) {
+			const int idx = blockIdx.x * kS1dPerWg + k;
+			if (idx < n && s_cc[k] > best) { best = s_cc[k]; bi = idx; }
+		}
+		part[(long long)blockIdx.y * ldp + blockIdx.x] = make_float2(best, __int_as_float(bi));
+	}
+}
+
+// First strict maximum of the stage (CFindDefocus1D.cpp:97-111: phase
+// outer, defocus inner, `>`) from the workgroups' maxima, then the next
+// stage: mFindDefocus hands its defocus and phase to mRefineDefocus
+// (CFindCtf1D.cpp:73-92); the refine stage's maximum is the result (dfMin =
+// dfMax, score = its CC).  One workgroup of 256 per micrograph.
+__global__ __launch_bounds__(256) void k_s1d_pick(SearchState* __restrict__ st, const ImgPlan* __restrict__ plans,
+	int ldp, const float2* __restrict__ part)
+{
+	__shared__ float s_cc[256];
+	__shared__ int s_i[256];
+	SearchState& S = st[blockIdx.x];
+	if (S.skip || S.s1_stage > 1) return;
+	const float2* pp = part + (long long)blockIdx.x * ldp;
+	const int nwg = (S.s1_n + kS1dPerWg - 1) / kS1dPerWg;
 	float best = (float)-1e20;
 	int bi = -1;
-	for (int i = threadIdx.x; i < npts; i += blockDim.x) {
-		int ifo = i % ndf, iph = i / ndf;
-		float df = dfr0 + (float)ifo * dfstep;
-		float ph = phr0 + (float)iph * psstep;
-		if (ph < 0) ph = 0.0f;
-		else if (ph > 150.0f) ph = 150.0f;
-		float cc = cc1d_eval(p, rad, w1d, df, ph);
-		if (cc > best) { best = cc; bi = i; }
+	for (int k = threadIdx.x; k < nwg; k += 256) {
+		const float2 v = pp[k];
+		const int i = __float_as_int(v.y);
+		if (i >= 0 && v.x > best) { best = v.x; bi = i; }
 	}
 	s_cc[threadIdx.x] = best;
 	s_i[threadIdx.x] = bi;
 	__syncthreads();
-	for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-		if (threadIdx.x < s) {
-			float c2 = s_cc[threadIdx.x + s];
-			int i2 = s_i[threadIdx.x + s];
-			float c1 = s_cc[threadIdx.x];
-			int i1 = s_i[threadIdx.x];
+	for (int o = 128; o > 0; o >>= 1) {
+		if (threadIdx.x < o) {
+			const float c2 = s_cc[threadIdx.x + o], c1 = s_cc[threadIdx.x];
+			const int i2 = s_i[threadIdx.x + o], i1 = s_i[threadIdx.x];
 			if (i2 >= 0 && (i1 < 0 || c2 > c1 || (c2 == c1 && i2 < i1))) {
 				s_cc[threadIdx.x] = c2; s_i[threadIdx.x] = i2;
 			}
 		}
 		__syncthreads();
 	}
-	if (threadIdx.x == 0) {
-		int i = s_i[0];
-		float r0 = 0.f, r1 = 0.f, r2 = (float)-1e20;
-		if (i >= 0) {
-			int ifo = i % ndf, iph = i / ndf;
-			r0 = dfr0 + (float)ifo * dfstep;
-			float ph = phr0 + (float)iph * psstep;
-			if (ph < 0) ph = 0.0f;
-			else if (ph > 150.0f) ph = 150.0f;
-			r1 = ph; r2 = s_cc[0];
-		}
-		out3[0] = r0; out3[1] = r1; out3[2] = r2;
+	if (threadIdx.x != 0) return;
+	const int i = s_i[0];
+	float df = 0.f, ph = 0.f, score = (float)-1e20;
+	if (i >= 0) {
+		s1d_candidate(S, i, df, ph);
+		score = s_cc[0];
 	}
-	__syncthreads();
-}
-
-__global__ __launch_bounds__(256) void k_search1d(Search1DParams p0, const ImgPlan* __restrict__ plans,
-	const float* __restrict__ radial, const double* __restrict__ w1d, SearchState* __restrict__ st)
-{
-	__shared__ float s_cc[256];
-	__shared__ int s_i[256];
-	__shared__ float res[3];
-	SearchState* S = st + blockIdx.x;
-	if (S->skip) return;
-	Search1DParams p = p0;
-	p.c = plans[blockIdx.x].cc;
-	p.lo = plans[blockIdx.x].lo1d;
-	p.hi = plans[blockIdx.x].hi1d;
-	const float* rad = radial + (long long)blockIdx.x * p.cmpx;
-	const float pix = p.c.pix;
-	float df_min, ext_phase;
-	if (p.mode == 0) {
-		// CFindCtf1D::mFindDefocus
-		float p2 = pix * pix;
-		float dfr0 = p.scan_lo > 0.0f ? p.scan_lo : 3000.0f * p2;
-		float dfr1 = p.scan_hi > 0.0f ? p.scan_hi : 30000.0f * p2;
-		float phr0 = fmaxf(S->ext_phase - S->phase_range / 2, 0.0f);
-		float phr1 = fminf(phr0 + S->phase_range, 150.0f);
-		search1d_brute(p, rad, w1d, dfr0, dfr1, phr0, phr1, p.scan_step, res, s_cc, s_i);
-		df_min = res[0];
-		ext_phase = res[1];
-	} else {
-		// CFindCtf1D::Refine1D
-		df_min = p.init_df;
-		ext_phase = S->ext_phase;
+	if (S.s1_stage == 0) {
+		// CFindCtf1D::Do1D: refine over max(0.3 df, 3000) A around the scan's best
+		s1d_refine_lattice(S, df, ph, fmaxf(0.3f * df, 3000.0f), plans[blockIdx.x].cc.pix);
+		return;
 	}
-	// CFindCtf1D::mRefineDefocus
-	float range = p.mode == 0 ? fmaxf(0.3f * df_min, 3000.0f) : p.range;
-	float mindf = 3000.0f * pix * pix;
-	float dfr0 = fmaxf(df_min - range / 2, mindf);
-	float dfr1 = dfr0 + range;
-	float prange = 0.2f * S->phase_range;
-	float phr0 = fmaxf(ext_phase - prange / 2, 0.0f);
-	float phr1 = fminf(0.0f + prange, 150.0f);
-	search1d_brute(p, rad, w1d, dfr0, dfr1, phr0, phr1, 0.0f, res, s_cc, s_i);
-	if (threadIdx.x == 0) {
-		S->ext_phase = res[1];
-		S->df_min = res[0];
-		S->df_max = res[0];
-		S->score = res[2];
-	}
+	S.ext_phase = ph;
+	S.df_min = df;
+	S.df_max = df;
+	S.score = score;
+	S.s1_stage = 2;
 }
 
 // ---------------------------------------------------------------------------

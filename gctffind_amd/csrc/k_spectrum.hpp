@@ -74,17 +74,16 @@ __device__ __forceinline__ float wave_row_sum(float v)
 __host__ __device__ constexpr int rescale_waves(int maxn) { return maxn <= 8192 ? 8 : 4; }
 
 // Real rows -> half-spectrum rows kept for kx < ncx_out.  Two rows share one
-// complex transform (z = a + i b).  Grid (ceil(ny/2), images).
+// complex transform (z = a + i b).  Row pair bx of one image.
 template <typename Tin, int NT, int MAXN>
-__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_r2c(const Tin* __restrict__ img,
-	long long img_stride, int nx, int ny, Fft1dPlan plan, int ncx_out,
-	float2* __restrict__ out, long long out_stride)
+__device__ __forceinline__ void rows_r2c_body(const Tin* __restrict__ img, int nx, int ny, const Fft1dPlan& plan,
+	int ncx_out, float2* __restrict__ out, int bx)
 {
 	__shared__ float2 buf[fft_lds_len(MAXN)];
 	const int tid = threadIdx.x;
-	const int y0 = 2 * blockIdx.x;
+	const int y0 = 2 * bx;
 	const bool two = (y0 + 1) < ny;
-	const Tin* src = img + blockIdx.y * img_stride + (long long)y0 * nx;
+	const Tin* src = img + (long long)y0 * nx;
 	for (int x = tid; x < nx; x += NT) {
 		float a = load_px(src, x);
 		float b = two ? load_px(src, (long long)nx + x) : 0.f;
@@ -92,7 +91,7 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_r2c(co
 	}
 	__syncthreads();
 	fft_any_lds<NT, MAXN, false>(buf, plan, tid);
-	float2* dst = out + blockIdx.y * out_stride + (long long)y0 * ncx_out;
+	float2* dst = out + (long long)y0 * ncx_out;
 	for (int k = tid; k < ncx_out; k += NT) {
 		float2 z = buf[pidx(k)];
 		float2 zc = buf[pidx(k == 0 ? 0 : nx - k)];
@@ -104,25 +103,34 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_r2c(co
 	}
 }
 
+// Grid (ceil(ny/2), images).
+template <typename Tin, int NT, int MAXN>
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_r2c(const Tin* __restrict__ img,
+	long long img_stride, int nx, int ny, Fft1dPlan plan, int ncx_out,
+	float2* __restrict__ out, long long out_stride)
+{
+	rows_r2c_body<Tin, NT, MAXN>(img + blockIdx.y * img_stride, nx, ny, plan, ncx_out,
+		out + blockIdx.y * out_stride, blockIdx.x);
+}
+
 // Rows too long for one LDS transform of length nx (nx even, not a power of
 // two, Bluestein length above MAXN): one row per workgroup as a complex
 // transform of length h = nx/2 of z[m] = x[2m] + i x[2m+1].  With
 // E = (Z[k] + conj Z[h-k]) / 2 and O = -i (Z[k] - conj Z[h-k]) / 2 the
 // transforms of the even and odd samples, X[k] = E[k] + exp(-2 pi i k/nx) O[k]
-// for k < ncx_out <= h.  plan is the length-h plan.  Grid (ny, images).
+// for k < ncx_out <= h.  plan is the length-h plan.  Row y of one image.
 template <typename Tin, int NT, int MAXN>
-__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_r2c_half(const Tin* __restrict__ img,
-	long long img_stride, int nx, int ny, Fft1dPlan plan, int ncx_out,
-	float2* __restrict__ out, long long out_stride)
+__device__ __forceinline__ void rows_r2c_half_body(const Tin* __restrict__ img, int nx, const Fft1dPlan& plan,
+	int ncx_out, float2* __restrict__ out, int y)
 {
 	__shared__ float2 buf[fft_lds_len(MAXN)];
 	const int tid = threadIdx.x;
 	const int h = nx / 2;
-	const Tin* src = img + blockIdx.y * img_stride + (long long)blockIdx.x * nx;
+	const Tin* src = img + (long long)y * nx;
 	for (int m = tid; m < h; m += NT) buf[pidx(m)] = make_float2(load_px(src, 2 * m), load_px(src, 2 * m + 1));
 	__syncthreads();
 	fft_any_lds<NT, MAXN, false>(buf, plan, tid);
-	float2* dst = out + blockIdx.y * out_stride + (long long)blockIdx.x * ncx_out;
+	float2* dst = out + (long long)y * ncx_out;
 	for (int k = tid; k < ncx_out; k += NT) {
 		const float2 z = buf[pidx(k)];
 		const float2 zc = buf[pidx(k == 0 ? 0 : h - k)];
@@ -134,14 +142,24 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_r2c_ha
 	}
 }
 
-// Complex transpose out[x][y] = in[y][x]; in is rows x cols.  32x32 tiles.
-__global__ __launch_bounds__(256) void k_transpose_c(const float2* __restrict__ in,
-	long long in_stride, int rows, int cols, float2* __restrict__ out, long long out_stride)
+// Grid (ny, images).
+template <typename Tin, int NT, int MAXN>
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_r2c_half(const Tin* __restrict__ img,
+	long long img_stride, int nx, int ny, Fft1dPlan plan, int ncx_out,
+	float2* __restrict__ out, long long out_stride)
+{
+	(void)ny;
+	rows_r2c_half_body<Tin, NT, MAXN>(img + blockIdx.y * img_stride, nx, plan, ncx_out,
+		out + blockIdx.y * out_stride, blockIdx.x);
+}
+
+// Complex transpose out[x][y] = in[y][x]; in is rows x cols.  32x32 tiles;
+// tile (bx, by) of one image.
+__device__ __forceinline__ void transpose_body(const float2* __restrict__ src, int rows, int cols,
+	float2* __restrict__ dst, int bxi, int byi)
 {
 	__shared__ float2 t[32][33];
-	const float2* src = in + blockIdx.z * in_stride;
-	float2* dst = out + blockIdx.z * out_stride;
-	const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+	const int bx = bxi * 32, by = byi * 32;
 	const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // ty in [0, 8)
 	for (int j = ty; j < 32; j += 8) {
 		int y = by + j, x = bx + tx;
@@ -154,20 +172,23 @@ __global__ __launch_bounds__(256) void k_transpose_c(const float2* __restrict__ 
 	}
 }
 
+__global__ __launch_bounds__(256) void k_transpose_c(const float2* __restrict__ in,
+	long long in_stride, int rows, int cols, float2* __restrict__ out, long long out_stride)
+{
+	transpose_body(in + blockIdx.z * in_stride, rows, cols, out + blockIdx.z * out_stride, blockIdx.x, blockIdx.y);
+}
+
 // Columns (rows of the transposed spectrum in[kx][y]): forward length ny,
 // crop to nny rows (ky <= nny/2 from the top, the rest from the bottom),
 // inverse length nny, written back row-major: out[y'][kx] (ncx entries per
-// y').  Grid (8 ceil(ncx / 8), images).
+// y').  Column kx of one image.
 template <int NT, int MAXN>
-__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_cols(const float2* __restrict__ in,
-	long long in_stride, int ny, int nny, int ncx, Fft1dPlan pf, Fft1dPlan pi,
-	float2* __restrict__ out, long long out_stride)
+__device__ __forceinline__ void cols_body(const float2* __restrict__ in, int ny, int nny, int ncx,
+	const Fft1dPlan& pf, const Fft1dPlan& pi, float2* __restrict__ out, int kx)
 {
 	__shared__ float2 buf[fft_lds_len(MAXN)];
 	const int tid = threadIdx.x;
-	const int kx = xcd_group(blockIdx.x, ncx);
-	if (kx < 0) return;
-	const float2* src = in + blockIdx.y * in_stride + (long long)kx * ny;
+	const float2* src = in + (long long)kx * ny;
 	for (int y = tid; y < ny; y += NT) buf[pidx(y)] = src[y];
 	__syncthreads();
 	fft_any_lds<NT, MAXN, false>(buf, pf, tid);
@@ -177,8 +198,19 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_cols(const 
 		return buf[pidx(ys)];
 	};
 	fft_any_lds_src<NT, MAXN, true>(buf, pi, tid, crop);
-	float2* dst = out + blockIdx.y * out_stride + kx;
+	float2* dst = out + kx;
 	for (int y = tid; y < nny; y += NT) dst[(long long)y * ncx] = buf[pidx(y)];
+}
+
+// Grid (8 ceil(ncx / 8), images).
+template <int NT, int MAXN>
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_cols(const float2* __restrict__ in,
+	long long in_stride, int ny, int nny, int ncx, Fft1dPlan pf, Fft1dPlan pi,
+	float2* __restrict__ out, long long out_stride)
+{
+	const int kx = xcd_group(blockIdx.x, ncx);
+	if (kx < 0) return;
+	cols_body<NT, MAXN>(in + blockIdx.y * in_stride, ny, nny, ncx, pf, pi, out + blockIdx.y * out_stride, kx);
 }
 
 // Columns too long for one LDS transform (ny even, not a power of two, above
@@ -187,20 +219,17 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_cols(const 
 // h = ny/2), X[k] = E[k mod h] + exp(-2 pi i k/ny) O[k mod h].  Each thread
 // parks the E values of its kept rows y' in the output column (out[y'][kx])
 // until O is done, and reads them back itself.  pf is the length-h plan;
-// nny <= MAXN/2.  Grid (8 ceil(ncx / 8), images).
+// nny <= MAXN/2.  Column kx of one image.
 template <int NT, int MAXN>
-__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_cols_split(const float2* __restrict__ in,
-	long long in_stride, int ny, int nny, int ncx, Fft1dPlan pf, Fft1dPlan pi,
-	float2* __restrict__ out, long long out_stride)
+__device__ __forceinline__ void cols_split_body(const float2* __restrict__ in, int ny, int nny, int ncx,
+	const Fft1dPlan& pf, const Fft1dPlan& pi, float2* __restrict__ out, int kx)
 {
 	constexpr int JN = (MAXN / 2 + NT - 1) / NT;     // kept rows per thread
 	__shared__ float2 buf[fft_lds_len(MAXN)];
 	const int tid = threadIdx.x;
-	const int kx = xcd_group(blockIdx.x, ncx);
-	if (kx < 0) return;
 	const int h = ny / 2;
-	const float2* src = in + blockIdx.y * in_stride + (long long)kx * ny;
-	float2* dst = out + blockIdx.y * out_stride + kx;
+	const float2* src = in + (long long)kx * ny;
+	float2* dst = out + kx;
 	for (int m = tid; m < h; m += NT) buf[pidx(m)] = src[2 * m];
 	__syncthreads();
 	fft_any_lds<NT, MAXN, false>(buf, pf, tid);
@@ -241,26 +270,36 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_cols_split(
 	for (int y = tid; y < nny; y += NT) dst[(long long)y * ncx] = buf[pidx(y)];
 }
 
+// Grid (8 ceil(ncx / 8), images).
+template <int NT, int MAXN>
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_cols_split(const float2* __restrict__ in,
+	long long in_stride, int ny, int nny, int ncx, Fft1dPlan pf, Fft1dPlan pi,
+	float2* __restrict__ out, long long out_stride)
+{
+	const int kx = xcd_group(blockIdx.x, ncx);
+	if (kx < 0) return;
+	cols_split_body<NT, MAXN>(in + blockIdx.y * in_stride, ny, nny, ncx, pf, pi, out + blockIdx.y * out_stride, kx);
+}
+
 // Half-spectrum rows -> real rows of length nnx (C2R; imaginary parts of the
 // DC and Nyquist bins are dropped, i.e. the real part of the inverse of the
 // Hermitian extension).  Two rows per transform.  Output rows have `pitch`
-// floats; the pad columns are zeroed.  Grid (ceil(nny/2), images).
+// floats; the pad columns are zeroed.  Row pair bx of one image.
 //
 // With rowsum, the row segments of the tile grid's mean blocks are summed
 // too (one wavefront per block column, FP32, fixed order):
-// rowsum[image][y - offy][bi] for the block rows y - offy in
+// rowsum[y - offy][bi] for the block rows y - offy in
 // [0, (nby-1) step + bsize).  k_block_rows turns them into block sums.
 template <int NT, int MAXN>
-__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r(const float2* __restrict__ in,
-	long long in_stride, int nny, int nnx, Fft1dPlan plan, float* __restrict__ out,
-	long long out_stride, int pitch, TileGrid g, float* __restrict__ rowsum)
+__device__ __forceinline__ void rows_c2r_body(const float2* __restrict__ in, int nny, int nnx,
+	const Fft1dPlan& plan, float* __restrict__ out, int pitch, const TileGrid& g, float* __restrict__ rowsum, int bx)
 {
 	__shared__ float2 buf[fft_lds_len(MAXN)];
 	const int tid = threadIdx.x;
 	const int ncx = nnx / 2 + 1;
-	const int y0 = 2 * blockIdx.x;
+	const int y0 = 2 * bx;
 	const bool two = (y0 + 1) < nny;
-	const float2* ga = in + blockIdx.y * in_stride + (long long)y0 * ncx;
+	const float2* ga = in + (long long)y0 * ncx;
 	const float2* gb = ga + ncx;
 	// Z = A + i B of the Hermitian extensions, read straight into the
 	// transform's first pass
@@ -278,7 +317,7 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r(co
 		return make_float2(a.x - b.y, a.y + b.x);
 	};
 	fft_any_lds_src<NT, MAXN, true>(buf, plan, tid, herm);
-	float* da = out + blockIdx.y * out_stride + (long long)y0 * pitch;
+	float* da = out + (long long)y0 * pitch;
 	for (int x = tid; x < pitch; x += NT) {
 		float2 z = x < nnx ? buf[pidx(x)] : make_float2(0.f, 0.f);
 		da[x] = z.x;
@@ -288,7 +327,6 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r(co
 	const int nr = (g.nby - 1) * g.step + g.bsize;
 	const int ra = y0 - g.offy;
 	if (ra + 1 < 0 || ra >= nr) return;
-	float* rs = rowsum + (long long)blockIdx.y * nr * g.nbx;
 	const int wave = tid >> 6, lane = tid & 63;
 	for (int bi = wave; bi < g.nbx; bi += NT / 64) {
 		const int x0 = g.offx + bi * g.step;
@@ -301,10 +339,22 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r(co
 		sa = wave_row_sum(sa);
 		sb = wave_row_sum(sb);
 		if (lane == 0) {
-			if (ra >= 0) rs[ra * g.nbx + bi] = sa;
-			if (two && ra + 1 < nr) rs[(ra + 1) * g.nbx + bi] = sb;
+			if (ra >= 0) rowsum[ra * g.nbx + bi] = sa;
+			if (two && ra + 1 < nr) rowsum[(ra + 1) * g.nbx + bi] = sb;
 		}
 	}
+}
+
+// Grid (ceil(nny/2), images); rowsum [image][block rows][nbx] or null.
+template <int NT, int MAXN>
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r(const float2* __restrict__ in,
+	long long in_stride, int nny, int nnx, Fft1dPlan plan, float* __restrict__ out,
+	long long out_stride, int pitch, TileGrid g, float* __restrict__ rowsum)
+{
+	float* rs = rowsum == nullptr ? nullptr
+		: rowsum + (long long)blockIdx.y * ((g.nby - 1) * g.step + g.bsize) * g.nbx;
+	rows_c2r_body<NT, MAXN>(in + blockIdx.y * in_stride, nny, nnx, plan, out + blockIdx.y * out_stride, pitch, g,
+		rs, blockIdx.x);
 }
 
 // C2R rows too long for one LDS transform of length nnx (as for the R2C rows
@@ -312,17 +362,15 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r(co
 // extension of the half row, the inverse transform of length h of
 // F[k] = (X[k] + X[k+h]) + i exp(2 pi i k/nnx) (X[k] - X[k+h]) is
 // z[m] = x[2m] + i x[2m+1].  plan is the length-h plan.  Row partials as in
-// k_rescale_rows_c2r (same pixels, same order).  Grid (nny, images).
+// k_rescale_rows_c2r (same pixels, same order).  Row y of one image.
 template <int NT, int MAXN>
-__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r_half(const float2* __restrict__ in,
-	long long in_stride, int nny, int nnx, Fft1dPlan plan, float* __restrict__ out,
-	long long out_stride, int pitch, TileGrid g, float* __restrict__ rowsum)
+__device__ __forceinline__ void rows_c2r_half_body(const float2* __restrict__ in, int nnx, const Fft1dPlan& plan,
+	float* __restrict__ out, int pitch, const TileGrid& g, float* __restrict__ rowsum, int y)
 {
 	__shared__ float2 buf[fft_lds_len(MAXN)];
 	const int tid = threadIdx.x;
 	const int ncx = nnx / 2 + 1, h = nnx / 2;
-	const int y = blockIdx.x;
-	const float2* ga = in + blockIdx.y * in_stride + (long long)y * ncx;
+	const float2* ga = in + (long long)y * ncx;
 	auto fold = [&](int k) -> float2 {
 		float2 a = ga[k];                              // X[k], k < h
 		float2 b = ga[h - k];                          // X[k+h] = conj X[h-k]
@@ -336,7 +384,7 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r_ha
 		return make_float2(su.x - t.y, su.y + t.x);    // su + i t
 	};
 	fft_any_lds_src<NT, MAXN, true>(buf, plan, tid, fold);
-	float* da = out + blockIdx.y * out_stride + (long long)y * pitch;
+	float* da = out + (long long)y * pitch;
 	for (int x = tid; x < pitch; x += NT) {
 		float v = 0.f;
 		if (x < nnx) {
@@ -349,7 +397,6 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r_ha
 	const int nr = (g.nby - 1) * g.step + g.bsize;
 	const int ra = y - g.offy;
 	if (ra < 0 || ra >= nr) return;
-	float* rs = rowsum + (long long)blockIdx.y * nr * g.nbx;
 	const int wave = tid >> 6, lane = tid & 63;
 	for (int bi = wave; bi < g.nbx; bi += NT / 64) {
 		const int x0 = g.offx + bi * g.step;
@@ -360,8 +407,110 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r_ha
 			sa += (x & 1) ? z.y : z.x;
 		}
 		sa = wave_row_sum(sa);
-		if (lane == 0) rs[ra * g.nbx + bi] = sa;
+		if (lane == 0) rowsum[ra * g.nbx + bi] = sa;
 	}
+}
+
+// Grid (nny, images).
+template <int NT, int MAXN>
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r_half(const float2* __restrict__ in,
+	long long in_stride, int nny, int nnx, Fft1dPlan plan, float* __restrict__ out,
+	long long out_stride, int pitch, TileGrid g, float* __restrict__ rowsum)
+{
+	(void)nny;
+	float* rs = rowsum == nullptr ? nullptr
+		: rowsum + (long long)blockIdx.y * ((g.nby - 1) * g.step + g.bsize) * g.nbx;
+	rows_c2r_half_body<NT, MAXN>(in + blockIdx.y * in_stride, nnx, plan, out + blockIdx.y * out_stride, pitch, g,
+		rs, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Rescale over a work list of micrographs of different shapes, types and
+// pixel sizes: each stage runs as one launch per kernel class (transform
+// variant and length) over the workgroup items of every micrograph of that
+// class, instead of one launch per micrograph.  The bodies are the
+// single-image kernels', so both give the same bits.
+// ---------------------------------------------------------------------------
+struct RescaleJob {
+	const void* img;          // raw pixels (the class launch's Tin)
+	int nx, ny;               // raw size
+	int nnx, nny, ncx;        // rescaled size, kept columns nnx/2 + 1
+	int pitch;                // padded output row (floats)
+	Fft1dPlan rows, cf, ci, c2r;
+	float2* A;                // ny x ncx row spectra, then nny x ncx rescaled columns
+	float2* B;                // ncx x ny transposed spectrum
+	float* out;               // padded rescaled image
+	float* rowsum;            // mean-block row partials of g (null: none)
+	TileGrid g;
+};
+
+// Launch item `item` belongs to entry e with first[e] <= item < first[e+1];
+// entry e is micrograph job[e].
+__device__ __forceinline__ int find_entry(const int* __restrict__ first, int n, int item)
+{
+	int lo = 0, hi = n - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (first[mid] <= item) lo = mid;
+		else hi = mid - 1;
+	}
+	return lo;
+}
+
+// Rows R2C (pairs, or half-length rows with HALF): items ceil(ny/2) or ny.
+template <typename Tin, int NT, int MAXN, bool HALF>
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_r2c_list(
+	const RescaleJob* __restrict__ jobs, const int* __restrict__ first, const int* __restrict__ job, int n)
+{
+	const int e = find_entry(first, n, blockIdx.x);
+	const RescaleJob& J = jobs[job[e]];
+	const int b = blockIdx.x - first[e];
+	if constexpr (HALF)
+		rows_r2c_half_body<Tin, NT, MAXN>((const Tin*)J.img, J.nx, J.rows, J.ncx, J.A, b);
+	else
+		rows_r2c_body<Tin, NT, MAXN>((const Tin*)J.img, J.nx, J.ny, J.rows, J.ncx, J.A, b);
+}
+
+// Transpose A (ny x ncx) -> B: items ceil(ncx/32) ceil(ny/32).
+__global__ __launch_bounds__(256) void k_transpose_list(const RescaleJob* __restrict__ jobs,
+	const int* __restrict__ first, const int* __restrict__ job, int n)
+{
+	const int e = find_entry(first, n, blockIdx.x);
+	const RescaleJob& J = jobs[job[e]];
+	const int b = blockIdx.x - first[e], tx = (J.ncx + 31) / 32;
+	transpose_body(J.A, J.ny, J.ncx, J.B, b % tx, b / tx);
+}
+
+// Columns B -> A (SPLIT: even/odd forward halves): items 8 ceil(ncx/8), so
+// every micrograph's first item is a multiple of 8 and xcd_group sees the
+// hardware's XCD of the workgroup.
+template <int NT, int MAXN, bool SPLIT>
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_cols_list(
+	const RescaleJob* __restrict__ jobs, const int* __restrict__ first, const int* __restrict__ job, int n)
+{
+	const int e = find_entry(first, n, blockIdx.x);
+	const RescaleJob& J = jobs[job[e]];
+	const int kx = xcd_group(blockIdx.x - first[e], J.ncx);
+	if (kx < 0) return;
+	if constexpr (SPLIT)
+		cols_split_body<NT, MAXN>(J.B, J.ny, J.nny, J.ncx, J.cf, J.ci, J.A, kx);
+	else
+		cols_body<NT, MAXN>(J.B, J.ny, J.nny, J.ncx, J.cf, J.ci, J.A, kx);
+}
+
+// Rows C2R A -> padded image with the mean-block row partials (HALF:
+// half-length rows): items ceil(nny/2) or nny.
+template <int NT, int MAXN, bool HALF>
+__global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_c2r_list(
+	const RescaleJob* __restrict__ jobs, const int* __restrict__ first, const int* __restrict__ job, int n)
+{
+	const int e = find_entry(first, n, blockIdx.x);
+	const RescaleJob& J = jobs[job[e]];
+	const int b = blockIdx.x - first[e];
+	if constexpr (HALF)
+		rows_c2r_half_body<NT, MAXN>(J.A, J.nnx, J.c2r, J.out, J.pitch, J.g, J.rowsum, b);
+	else
+		rows_c2r_body<NT, MAXN>(J.A, J.nny, J.nnx, J.c2r, J.out, J.pitch, J.g, J.rowsum, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -582,6 +731,29 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_ro
 		blockIdx.x - J.rows_first);
 }
 
+// Persistent form over a tile work list: `gridDim.x` workgroups take items
+// 0 .. gridDim.x - 1 first, then pull the next item of the list from a work
+// queue (*next, an atomic counter zeroed before the launch) until all
+// `items` are done.  Same items and arithmetic as k_tile_rows_list.
+template <typename Tin, int T, int ROWS, int G>
+__global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_rows_queue(
+	const TileJob* __restrict__ jobs, int njobs, int items, int* __restrict__ next,
+	const double* __restrict__ sums, const float* __restrict__ mask, const float2* __restrict__ tw,
+	float2* __restrict__ rt)
+{
+	__shared__ int s_item;
+	int item = blockIdx.x;
+	while (item < items) {
+		const TileJob& J = jobs[find_job(jobs, njobs, item)];
+		tile_rows_body<Tin, T, ROWS, G>((const Tin*)J.img, J.pitch, J.g, sums + J.sums_off, mask, tw, rt + J.rt_off,
+			item - J.rows_first);
+		__syncthreads();
+		if (threadIdx.x == 0) s_item = (int)gridDim.x + atomicAdd(next, 1);
+		__syncthreads();
+		item = s_item;
+	}
+}
+
 // Column pass: for CPW kx-columns and the tiles [t0, t1) of split s,
 // transform along y, take |F| and accumulate tile * (1/ntiles).  Writes the
 // DC-shifted partial average part[image][s][y][kx].  Grid
@@ -650,6 +822,41 @@ __global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols_list(const TileJob*
 	const int ntiles = J.g.ntx * J.g.nty;
 	tile_cols_body<T, CPW>(rt + J.rt_off, ntiles, J.nsplit, 1.0f / (float)ntiles, tw, part + J.part_off,
 		blockIdx.x, blockIdx.y);
+}
+
+// Persistent form of k_tile_cols_list: item (j, s, cg) = (j nsplit_max + s)
+// ncg + cg for the jobs j, splits s < nsplit_max and column groups cg of
+// CPW columns; items of splits a job does not have are skipped.
+template <int T, int CPW>
+__global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols_queue(const TileJob* __restrict__ jobs, int njobs,
+	int nsplit_max, int* __restrict__ next, const float2* __restrict__ rt, const float2* __restrict__ tw,
+	float* __restrict__ part)
+{
+	constexpr int CX = T / 2 + 1;
+	constexpr int NCG = (CX + CPW - 1) / CPW;
+	const int items = njobs * nsplit_max * NCG;
+	__shared__ int s_item;
+	int item = blockIdx.x;
+	while (item < items) {
+		const int cg = item % NCG, r = item / NCG;
+		const int sp = r % nsplit_max, j = r / nsplit_max;
+		const TileJob& J = jobs[j];
+		if (sp < J.nsplit) {
+			const int ntiles = J.g.ntx * J.g.nty;
+			tile_cols_body<T, CPW>(rt + J.rt_off, ntiles, J.nsplit, 1.0f / (float)ntiles, tw, part + J.part_off, cg, sp);
+		}
+		__syncthreads();
+		if (threadIdx.x == 0) s_item = (int)gridDim.x + atomicAdd(next, 1);
+		__syncthreads();
+		item = s_item;
+	}
+}
+
+// Zero the work-queue counters of a list's persistent launches.
+__global__ void k_queue_reset(int* __restrict__ q, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) q[i] = 0;
 }
 
 // Sum the split partials in split order: raw[image][i] = sum_s part[s][i].

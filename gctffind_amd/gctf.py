@@ -133,7 +133,7 @@ def load():
         "gctf_ring_pixels": (ci, [vp, cf, cf, cf, fp, ci, fp]),
         "gctf_radial_average": (ci, [vp, vp, ci, vp, ci]),
         "gctf_estimate_batch": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, fp]),
-        "gctf_tune": (ci, [vp, ci, ci, ci]),
+        "gctf_tune": (ci, [vp, ci, ci, ci, ci, ci]),
         "gctf_profile": (ci, [vp, ci]),
         "gctf_profile_report": (ci, [vp, ci, fp, fp, fp, fp]),
         "gctf_grid_search_batch": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf,
@@ -141,7 +141,7 @@ def load():
         "gctf_do2d_batch_static": (ci, [vp, vp, ci, vp]),
         "gctf_estimate_list": (ci, [vp, ctypes.POINTER(GctfImage), ci, fp]),
         "gctf_estimate_series": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, fp, ci, fp]),
-        "gctf_autotune": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, fp, fp, fp]),
+        "gctf_autotune": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, fp, fp, fp, fp]),
         "gctf_learned_program_default": (ci, [ctypes.POINTER(GctfSearchProgram)]),
         "gctf_radial_batch": (ci, [vp, vp, ci, ci, ci, ci, ctypes.c_longlong, cf, vp, ci]),
         "gctf_set_proposal_params": (ci, [vp, fp, ctypes.c_longlong]),
@@ -346,12 +346,13 @@ class Context:
 
     def autotune(self, dev_ptr, n, nx, ny, pix, dtype=DTYPE_F32, stride=None):
         stride = nx * ny if stride is None else stride
-        c, sp, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        c, sp, ms, cb = ctypes.c_int(), ctypes.c_int(), ctypes.c_float(), ctypes.c_int()
         self.check(self.L.gctf_autotune(self.h, ctypes.c_void_p(dev_ptr), dtype, n, nx, ny, stride, pix,
                                         ctypes.cast(ctypes.pointer(c), ctypes.c_void_p),
                                         ctypes.cast(ctypes.pointer(sp), ctypes.c_void_p),
-                                        ctypes.cast(ctypes.pointer(ms), ctypes.c_void_p)))
-        return {"chunk": c.value, "tile_splits": sp.value, "ms": ms.value}
+                                        ctypes.cast(ctypes.pointer(ms), ctypes.c_void_p),
+                                        ctypes.cast(ctypes.pointer(cb), ctypes.c_void_p)))
+        return {"chunk": c.value, "tile_splits": sp.value, "ms": ms.value, "search_block": cb.value}
 
     def radial_batch(self, dev_ptr, n, nx, ny, pix, dev_out_ptr, ld, dtype=DTYPE_F32, stride=None):
         """Spectrum stages + radial average into a device buffer [n][ld]."""
@@ -375,8 +376,8 @@ class Context:
         out = [r.as_dict() for r in res]
         return (out, [p.as_dict() for p in props]) if proposals else out
 
-    def tune(self, chunk=0, tile_splits=0, use_graph=0):
-        self.check(self.L.gctf_tune(self.h, chunk, tile_splits, use_graph))
+    def tune(self, chunk=0, tile_splits=0, use_graph=0, search_block=0, list_queue=0):
+        self.check(self.L.gctf_tune(self.h, chunk, tile_splits, use_graph, search_block, list_queue))
 
     def profile(self, enable=True):
         self.check(self.L.gctf_profile(self.h, int(enable)))

@@ -188,8 +188,13 @@ GCTF_API int gctf_grid_search_batch(gctf_ctx* ctx, const void* dev_imgs, int dty
 	int* top_out, float* rescored_out);
 
 /* Knobs of the batched path (0 keeps the current value): images per
- * spectrum chunk, tile-column splits, graph use (1 on, -1 off). */
-GCTF_API int gctf_tune(gctf_ctx* ctx, int chunk, int tile_splits, int use_graph);
+ * spectrum chunk, tile-column splits, graph use (1 on, -1 off), 2D search
+ * blocking (candidates per evaluator workgroup: 4, 8 or 16) and the tile
+ * passes of gctf_estimate_list (1: persistent workgroups on a work queue,
+ * -1: one workgroup per item).  None of them changes results except the
+ * tile splits (FP32 partial-sum grouping of the tile average). */
+GCTF_API int gctf_tune(gctf_ctx* ctx, int chunk, int tile_splits, int use_graph, int search_block,
+	int list_queue);
 
 /* Kernel timing (no reference counterpart): while enabled, every kernel the
  * context launches is bracketed by HIP events on its stream and batches run
@@ -228,11 +233,14 @@ GCTF_API int gctf_estimate_series(gctf_ctx* ctx, const void* dev_imgs, int dtype
 	long long stride, float pix, const float* tilts_deg, int tomo, gctf_result* out);
 
 /* Time the spectrum stages of this batch for every (images per chunk, tile
- * splits) pair in {1, 2, 4, 8, 16} x {4, 8, 16} and keep the fastest for
- * this shape (nx, ny, dtype, pix): later batch, series and learned calls of
- * that shape use it.  Outputs optional. */
+ * splits) pair in {1, 2, 4, 8, 16} x {4, 8, 16}, then the plain search of
+ * its spectra with 4, 8 and 16 candidates per 2D-evaluator workgroup, and
+ * keep the fastest for this shape (nx, ny, dtype, pix): later batch,
+ * series, learned and list calls of that shape use them (results do not
+ * depend on the search blocking).  Outputs optional; best_ms is the
+ * spectrum stages' time. */
 GCTF_API int gctf_autotune(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, int nx, int ny,
-	long long stride, float pix, int* best_chunk, int* best_splits, float* best_ms);
+	long long stride, float pix, int* best_chunk, int* best_splits, float* best_ms, int* best_search_block);
 
 /* ---- learned-proposal path (no reference counterpart) -------------------
  * A 1D convolutional network reads each micrograph's radial spectrum (the

@@ -30,6 +30,9 @@ def test_uneven_list_equals_single_batches():
         (1280, 1024, 0.9, gctf.DTYPE_F16, 21000.0, 20500.0, -35.0),
         (1536, 1536, 1.0, gctf.DTYPE_F32, 9000.0, 8800.0, 60.0),
         (1024, 1200, 1.35, gctf.DTYPE_F16, 26000.0, 25000.0, -80.0),
+        # same shapes again: rescale classes with several micrographs
+        (1536, 1536, 1.0, gctf.DTYPE_F32, 12500.0, 12000.0, 5.0),
+        (1280, 1024, 0.9, gctf.DTYPE_F16, 17000.0, 16200.0, 75.0),
     ]
     bufs, images = [], []
     for i, (nx, ny, pix, dt, a, b, az) in enumerate(spec):
@@ -41,11 +44,37 @@ def test_uneven_list_equals_single_batches():
     ctx = gctf.Context()
     got = ctx.estimate_list(images)
     got2 = ctx.estimate_list(images)   # graph replay
+    cg = gctf.Context()
+    cg.tune(list_queue=-1)             # tile passes with one workgroup per item
+    got3 = cg.estimate_list(images)
     for i, im in enumerate(images):
         one = gctf.Context().estimate_batch(im["data"], 1, im["nx"], im["ny"], im["pix"], dtype=im["dtype"])[0]
         for k in ("df_min", "df_max", "azimuth", "score", "ctf_res"):
             assert got[i][k] == one[k], (i, k, got[i][k], one[k])
             assert got2[i][k] == one[k], (i, k)
+            assert got3[i][k] == one[k], (i, k)
+
+
+def test_search_blocking_keeps_results():
+    """4, 8 or 16 candidates per 2D-evaluator workgroup: identical results
+    (each candidate's sums are formed on their own)."""
+    gctf = _setup()
+    from gctffind_amd import synth
+    n = 3
+    imgs = torch.empty((n, 1024, 1024), dtype=torch.float32, device="cuda")
+    for i in range(n):
+        synth.micrograph(1024, 1024, pix=1.2, seed=450 + i, df1=10000.0 + 6000 * i + 300, df2=10000.0 + 6000 * i - 300,
+                         az_deg=-50.0 + 40 * i, device="cuda", out=imgs[i])
+    torch.cuda.synchronize()
+    res = {}
+    for cb in (4, 8, 16):
+        ctx = gctf.Context()
+        ctx.tune(search_block=cb)
+        res[cb] = ctx.estimate_batch(imgs.data_ptr(), n, 1024, 1024, 1.2)
+    for cb in (4, 16):
+        for i in range(n):
+            for k in ("df_min", "df_max", "azimuth", "score", "ctf_res"):
+                assert res[cb][i][k] == res[8][i][k], (cb, i, k)
 
 
 @pytest.mark.parametrize("tomo", [True, False])
@@ -117,6 +146,7 @@ def test_autotune_keeps_results():
     before = ctx.estimate_batch(imgs.data_ptr(), n, 1536, 1536, pix)
     best = ctx.autotune(imgs.data_ptr(), n, 1536, 1536, pix)
     assert best["chunk"] in (1, 2, 4) and best["tile_splits"] in (4, 8, 16) and best["ms"] > 0
+    assert best["search_block"] in (4, 8, 16)
     after = ctx.estimate_batch(imgs.data_ptr(), n, 1536, 1536, pix)
     # the chunk size does not touch the arithmetic; the tile-split count
     # regroups the FP32 tile sums, so results agree to the parity tolerance
@@ -126,3 +156,10 @@ def test_autotune_keeps_results():
         if best["tile_splits"] == 8:
             for k in ("df_min", "df_max", "azimuth", "score", "ctf_res"):
                 assert a[k] == b[k]
+    # a list of micrographs of the tuned shape takes the tuned tile splits:
+    # it equals the tuned batch bit for bit
+    lst = ctx.estimate_list([dict(data=imgs[i].data_ptr(), dtype=gctf.DTYPE_F32, nx=1536, ny=1536, pix=pix)
+                             for i in range(n)])
+    for a, b in zip(lst, after):
+        for k in ("df_min", "df_max", "azimuth", "score", "ctf_res"):
+            assert a[k] == b[k], k
