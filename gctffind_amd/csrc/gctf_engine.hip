@@ -822,7 +822,7 @@ int ensure_spectrum_buffers(gctf_ctx* c, const TileGrid& g, int T, int n)
 {
 	const size_t cx = T / 2 + 1;
 	const size_t ntiles = (size_t)g.ntx * g.nty;
-	CHK(ensure_buf(c, c->bsums, sizeof(double) * (size_t)g.nbx * g.nby * kSumParts * n));
+	CHK(ensure_buf(c, c->bsums, sizeof(float) * (size_t)g.nbx * g.nby * kSumParts * n));
 	const size_t nr = (size_t)(g.nby - 1) * g.step + g.bsize;   // rows of the mean blocks
 	CHK(ensure_buf(c, c->rowsum, sizeof(float) * nr * g.nbx * n));
 	CHK(ensure_buf(c, c->rt, sizeof(float2) * ntiles * cx * T * n));
@@ -848,13 +848,13 @@ int launch_spectrum_t(gctf_ctx* c, const TileGrid& g, const TileTables& tt, cons
 	}
 	const int np = g.nbx * g.nby * kSumParts;
 	PROF(c, "k_block_rows", k_block_rows<<<dim3((np + 255) / 256, n), 256, 0, s>>>(c->rowsum.as<float>(), g,
-		c->bsums.as<double>()));
+		c->bsums.as<float>()));
 	constexpr int ROWS = 16;                                 // rows per row-pass workgroup
 	const long long rts = (long long)ntiles * CX * T;
 	constexpr int RG = 2;                                    // row blocks per row-pass workgroup
 	PROF(c, "k_tile_rows", k_tile_rows<Tin, T, ROWS, RG><<<dim3((ntiles * (T / ROWS) + RG - 1) / RG, n),
 		(ROWS / 2) * (T / 8), 0, s>>>(
-		img, stride, pitch, g, c->bsums.as<double>(), tt.mask.as<float>(), tt.tw.as<float2>(),
+		img, stride, pitch, g, c->bsums.as<float>(), tt.mask.as<float>(), tt.tw.as<float2>(),
 		c->rt.as<float2>(), rts));
 	constexpr int CPW = 4;
 	const float fac = 1.0f / (float)ntiles;                   // CGenAvgSpectrum.cpp:90
@@ -2450,20 +2450,20 @@ int launch_tile_list_t(gctf_ctx* c, const TileTables& tt, const TileJob* d_jobs,
 	if (G.max_nr > 0)
 		PROF(c, "k_row_sums", k_row_sums_list<Tin><<<dim3(G.max_nr, G.nj), 256, 0, s>>>(jobs, c->rowsum.as<float>()));
 	PROF(c, "k_block_rows", k_block_rows_list<<<dim3((G.max_np + 255) / 256, G.nj), 256, 0, s>>>(jobs,
-		c->rowsum.as<float>(), c->bsums.as<double>()));
+		c->rowsum.as<float>(), c->bsums.as<float>()));
 	const int col_items = G.nj * G.max_split * NCG;
 	if (c->list_queue) {
 		int gr = 0, gc = 0;
 		CHK(resident_grid(c, k_tile_rows_queue<Tin, T, ROWS, RG>, (ROWS / 2) * (T / 8), G.items, &gr));
 		CHK(resident_grid(c, k_tile_cols_queue<T, CPW>, CPW * (T / 8), col_items, &gc));
 		PROF(c, "k_tile_rows_queue", k_tile_rows_queue<Tin, T, ROWS, RG><<<gr, (ROWS / 2) * (T / 8), 0, s>>>(jobs,
-			G.nj, G.items, qnext + G.q, c->bsums.as<double>(), tt.mask.as<float>(), tt.tw.as<float2>(),
+			G.nj, G.items, qnext + G.q, c->bsums.as<float>(), tt.mask.as<float>(), tt.tw.as<float2>(),
 			c->rt.as<float2>()));
 		PROF(c, "k_tile_cols_queue", k_tile_cols_queue<T, CPW><<<gc, CPW * (T / 8), 0, s>>>(jobs, G.nj, G.max_split,
 			qnext + G.q + 1, c->rt.as<float2>(), tt.tw.as<float2>(), c->tpart.as<float>()));
 	} else {
 		PROF(c, "k_tile_rows", k_tile_rows_list<Tin, T, ROWS, RG><<<G.items, (ROWS / 2) * (T / 8), 0, s>>>(jobs, G.nj,
-			c->bsums.as<double>(), tt.mask.as<float>(), tt.tw.as<float2>(), c->rt.as<float2>()));
+			c->bsums.as<float>(), tt.mask.as<float>(), tt.tw.as<float2>(), c->rt.as<float2>()));
 		PROF(c, "k_tile_cols", k_tile_cols_list<T, CPW><<<dim3(NCG, G.max_split, G.nj), CPW * (T / 8),
 			0, s>>>(jobs, c->rt.as<float2>(), tt.tw.as<float2>(), c->tpart.as<float>()));
 	}
@@ -2595,7 +2595,7 @@ int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result
 	CHK(ensure_buf(ctx, ctx->cbufB, sizeof(float2) * std::max<size_t>(most.cbuf, 1)));
 	CHK(ensure_buf(ctx, ctx->padc, sizeof(float) * std::max<size_t>(most.pad, 1)));
 	CHK(ensure_buf(ctx, ctx->rowsum, sizeof(float) * most.rowsum));
-	CHK(ensure_buf(ctx, ctx->bsums, sizeof(double) * most.sums));
+	CHK(ensure_buf(ctx, ctx->bsums, sizeof(float) * most.sums));
 	CHK(ensure_buf(ctx, ctx->rt, sizeof(float2) * most.rt));
 	CHK(ensure_buf(ctx, ctx->tpart, sizeof(float) * most.part));
 	CHK(ensure_buf(ctx, ctx->jobs, sizeof(TileJob) * (size_t)n));

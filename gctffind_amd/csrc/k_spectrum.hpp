@@ -568,10 +568,12 @@ __global__ __launch_bounds__(256) void k_row_sums_list(const TileJob* __restrict
 	row_sums_body<Tin>((const Tin*)J.img, J.pitch, J.g, rowsum + J.rowsum_off, blockIdx.x);
 }
 
-// Second step: part p of block (bi, bj) adds its rows' partials in order,
-// in FP64.  Grid (ceil(nbx nby kSumParts / 256), images).
+// Second step: part p of block (bi, bj) adds its rows' partials in FP32,
+// rows r0 + j into accumulator j % 4, then (a0 + a1) + (a2 + a3): a fixed
+// order whose rounding error grows with a quarter of the row count.  Grid
+// (ceil(nbx nby kSumParts / 256), images).
 __device__ __forceinline__ void block_rows_body(const float* __restrict__ rs, const TileGrid& g,
-	double* __restrict__ sums, int i)
+	float* __restrict__ sums, int i)
 {
 	const int nblk = g.nbx * g.nby;
 	if (i >= nblk * kSumParts) return;
@@ -579,13 +581,13 @@ __device__ __forceinline__ void block_rows_body(const float* __restrict__ rs, co
 	const int bi = b % g.nbx, bj = b / g.nbx;
 	const int rows = (g.bsize + kSumParts - 1) / kSumParts;
 	const int r0 = bj * g.step + p * rows, r1 = bj * g.step + min(g.bsize, (p + 1) * rows);
-	double acc = 0.0;
-	for (int r = r0; r < r1; ++r) acc += (double)rs[r * g.nbx + bi];
-	sums[(long long)b * kSumParts + p] = acc;
+	float a[4] = {0.f, 0.f, 0.f, 0.f};
+	for (int r = r0; r < r1; ++r) a[(r - r0) & 3] += rs[r * g.nbx + bi];
+	sums[(long long)b * kSumParts + p] = (a[0] + a[1]) + (a[2] + a[3]);
 }
 
 __global__ __launch_bounds__(256) void k_block_rows(const float* __restrict__ rowsum, TileGrid g,
-	double* __restrict__ sums)
+	float* __restrict__ sums)
 {
 	const int nr = (g.nby - 1) * g.step + g.bsize;
 	block_rows_body(rowsum + (long long)blockIdx.y * nr * g.nbx, g,
@@ -594,18 +596,18 @@ __global__ __launch_bounds__(256) void k_block_rows(const float* __restrict__ ro
 
 // Grid (ceil(largest nbx nby kSumParts / 256), jobs).
 __global__ __launch_bounds__(256) void k_block_rows_list(const TileJob* __restrict__ jobs,
-	const float* __restrict__ rowsum, double* __restrict__ sums)
+	const float* __restrict__ rowsum, float* __restrict__ sums)
 {
 	const TileJob& J = jobs[blockIdx.y];
 	block_rows_body(rowsum + J.rowsum_off, J.g, sums + J.sums_off, blockIdx.x * 256 + threadIdx.x);
 }
 
-__device__ __forceinline__ double block_sum(const double* bs, int b)
+// A block's sum from its kSumParts parts, pairwise in FP32.
+__device__ __forceinline__ float block_sum(const float* bs, int b)
 {
-	double s = 0.0;
-#pragma unroll
-	for (int p = 0; p < kSumParts; ++p) s += bs[b * kSumParts + p];
-	return s;
+	const float* q = bs + b * kSumParts;
+	static_assert(kSumParts == 8, "pairwise order written for 8 parts");
+	return ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
 }
 
 // Row pass of the tile FFT: for ROWS rows of one tile, subtract the tile
@@ -621,7 +623,7 @@ __device__ __forceinline__ double block_sum(const double* bs, int b)
 // and rt are the micrograph's pixels, block sums and row-pass output.
 template <typename Tin, int T, int ROWS, int G>
 __device__ __forceinline__ void tile_rows_body(const Tin* __restrict__ im, int pitch, const TileGrid& g,
-	const double* __restrict__ bs, const float* __restrict__ mask, const float2* __restrict__ tw,
+	const float* __restrict__ bs, const float* __restrict__ mask, const float2* __restrict__ tw,
 	float2* __restrict__ rt, int item)
 {
 	constexpr int FT = T / 8;              // threads per transform
@@ -661,14 +663,14 @@ __device__ __forceinline__ void tile_rows_body(const Tin* __restrict__ im, int p
 		if (b >= nblk) break;
 		const int rb = b % NRB, t = b / NRB;
 		const int tx = t % g.ntx, ty = t / g.ntx;
-		double tsum;
+		float tsum;
 		if (g.blocks) {
-			tsum = block_sum(bs, ty * g.nbx + tx) + block_sum(bs, ty * g.nbx + tx + 1)
-				+ block_sum(bs, (ty + 1) * g.nbx + tx) + block_sum(bs, (ty + 1) * g.nbx + tx + 1);
+			tsum = (block_sum(bs, ty * g.nbx + tx) + block_sum(bs, ty * g.nbx + tx + 1))
+				+ (block_sum(bs, (ty + 1) * g.nbx + tx) + block_sum(bs, (ty + 1) * g.nbx + tx + 1));
 		} else {
 			tsum = block_sum(bs, ty * g.nbx + tx);
 		}
-		const float mean = (float)(tsum / ((double)T * (double)T));
+		const float mean = tsum / ((float)T * (float)T);
 		const int ya = rb * ROWS + 2 * grp;
 #pragma unroll
 		for (int i = 0; i < PX; ++i) {
@@ -711,7 +713,7 @@ __device__ __forceinline__ void tile_rows_body(const Tin* __restrict__ im, int p
 
 template <typename Tin, int T, int ROWS, int G>
 __global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_rows(const Tin* __restrict__ img,
-	long long img_stride, int pitch, TileGrid g, const double* __restrict__ sums,
+	long long img_stride, int pitch, TileGrid g, const float* __restrict__ sums,
 	const float* __restrict__ mask, const float2* __restrict__ tw,
 	float2* __restrict__ rt, long long rt_stride)
 {
@@ -722,7 +724,7 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_ro
 // The same over a tile work list: grid (total items of the jobs).
 template <typename Tin, int T, int ROWS, int G>
 __global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_rows_list(
-	const TileJob* __restrict__ jobs, int njobs, const double* __restrict__ sums,
+	const TileJob* __restrict__ jobs, int njobs, const float* __restrict__ sums,
 	const float* __restrict__ mask, const float2* __restrict__ tw, float2* __restrict__ rt)
 {
 	const int j = find_job(jobs, njobs, blockIdx.x);
@@ -738,7 +740,7 @@ __global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_ro
 template <typename Tin, int T, int ROWS, int G>
 __global__ __launch_bounds__((ROWS / 2) * (T / 8), G > 1 ? 6 : 8) void k_tile_rows_queue(
 	const TileJob* __restrict__ jobs, int njobs, int items, int* __restrict__ next,
-	const double* __restrict__ sums, const float* __restrict__ mask, const float2* __restrict__ tw,
+	const float* __restrict__ sums, const float* __restrict__ mask, const float2* __restrict__ tw,
 	float2* __restrict__ rt)
 {
 	__shared__ int s_item;
@@ -913,7 +915,8 @@ __global__ __launch_bounds__(256) void k_bg_rows(const float* __restrict__ raw, 
 }
 
 // Column sums, the subtraction, the r < 0.04 zeroing (evaluated in float as
-// the reference does) and per-workgroup moment partials for the gate.
+// the reference does) and per-workgroup moment partials for the gate (an
+// FP32 tree within the workgroup; k_gate adds the partials in FP64).
 // Grid (ceil(T*CX/256), images).
 template <int T>
 __global__ __launch_bounds__(256) void k_bg_cols(const float* __restrict__ raw,
@@ -921,11 +924,11 @@ __global__ __launch_bounds__(256) void k_bg_cols(const float* __restrict__ raw,
 	double* __restrict__ mom)
 {
 	constexpr int CX = T / 2 + 1;
-	__shared__ double r1[256], r2[256];
+	__shared__ float r1[256], r2[256];
 	const int i = blockIdx.x * 256 + threadIdx.x;
 	const float* in = raw + (long long)blockIdx.z * T * CX;
 	const float* h = hsum + (long long)blockIdx.z * T * CX;
-	double m1 = 0.0, m2 = 0.0;
+	float m1 = 0.0f, m2 = 0.0f;
 	if (i < T * CX) {
 		const int y = i / CX, x = i % CX;
 		const int yc = y - T / 2;
@@ -943,7 +946,7 @@ __global__ __launch_bounds__(256) void k_bg_cols(const float* __restrict__ raw,
 			o = in[i] - acc;
 		}
 		spect[(long long)blockIdx.z * T * CX + i] = o;
-		m1 = o; m2 = (double)o * o;
+		m1 = o; m2 = o * o;
 	}
 	r1[threadIdx.x] = m1; r2[threadIdx.x] = m2;
 	__syncthreads();
@@ -1067,7 +1070,7 @@ __global__ __launch_bounds__(T / 8) void k_lp_y(const float* __restrict__ q,
 // Radial average (GRadialAvg.cu:11-33) from a host-built FP32 sample table.
 // ---------------------------------------------------------------------------
 // Grid (ceil(cmpx / 64), images), 64 threads: thread = bin, samples summed
-// in table order.
+// in table order in FP32.
 __global__ __launch_bounds__(64) void k_radial(const float* __restrict__ spect, int cmpx, int cmpy,
 	const int* __restrict__ roff, const int* __restrict__ rcnt, const int* __restrict__ ridx,
 	float* __restrict__ radial, int ld)
@@ -1075,10 +1078,11 @@ __global__ __launch_bounds__(64) void k_radial(const float* __restrict__ spect, 
 	const float* in = spect + (long long)blockIdx.y * cmpx * cmpy;
 	const int r = blockIdx.x * 64 + threadIdx.x;
 	if (r >= cmpx) return;
-	double s = 0.0;
+	// FP32 in the reference's sample order (GRadialAvg.cu:24-31)
+	float s = 0.0f;
 	const int o = roff[r], n = rcnt[r];
-	for (int i = 0; i < n; ++i) s += (double)in[ridx[o + i]];
-	radial[(long long)blockIdx.y * ld + r] = (float)(s / n);
+	for (int i = 0; i < n; ++i) s += in[ridx[o + i]];
+	radial[(long long)blockIdx.y * ld + r] = s / (float)n;
 }
 
 }  // namespace gctf

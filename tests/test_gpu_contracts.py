@@ -4,8 +4,9 @@ HIP path against the CPU oracle and the committed fixtures.
 
   * radial average: every one of the 257 bins equals the mean over the
     committed FP32 sample table (tests/golden/radial_table_257x512.npz) on
-    a spectrum of distinct pseudo-random values, so one sample read from a
-    different pixel moves its bin far beyond the 1e-6 tolerance
+    a spectrum of distinct pseudo-random values, within 5e-6 (the kernel
+    sums in FP32 in table order, as GRadialAvg.cu does: up to 180 terms);
+    one sample read from a different pixel moves its bin by about 2e-3
   * 2D correlation ring: the library's ring equals the committed index
     fixtures at 1.2 and 1.35 A (K = 47905 and 60793, the survey's counts)
   * flatness gate (CFindCtfBase.cpp:167-174): a spectrum with variance >= 1
@@ -76,10 +77,10 @@ def test_radial_average_every_bin_on_the_fixture_table():
     torch.cuda.synchronize()
     ctx.radial_average(d.data_ptr(), 2, out.data_ptr())
     got = out.cpu().numpy()
-    assert np.abs(got[0] - want).max() <= 1e-6 * np.abs(want).max(), np.argmax(np.abs(got[0] - want))
+    assert np.abs(got[0] - want).max() <= 5e-6 * np.abs(want).max(), np.argmax(np.abs(got[0] - want))
     # the oracle's FP64 average over its own table: the same bins
-    assert np.allclose(got[0], O.radial_avg(spect.astype(np.float64)), rtol=1e-6, atol=0)
-    assert np.allclose(got[1], O.radial_avg(spect[::-1].astype(np.float64)), rtol=1e-6, atol=0)
+    assert np.allclose(got[0], O.radial_avg(spect.astype(np.float64)), rtol=5e-6, atol=0)
+    assert np.allclose(got[1], O.radial_avg(spect[::-1].astype(np.float64)), rtol=5e-6, atol=0)
 
 
 @pytest.mark.parametrize("pix,count", [(1.2, 47905), (1.35, 60793)])
