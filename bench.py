@@ -354,8 +354,9 @@ def uneven_list(torch, gctf, ctx, n, steps, warmup):
 
 
 def evaluate(torch, ctx, a, buf, thr):
-    """Learned path against the plain path (the reference algorithm; its
-    defocus and azimuth equal the FP64 oracle's on the parity cases) on a.eval held-out micrographs,
+    """Learned path (confidence threshold thr) against the plain path (the
+    reference algorithm; its defocus and azimuth equal the FP64 oracle's on
+    the parity cases) on a.eval held-out micrographs,
     seeds 2000..: mean-defocus agreement overall, inside the reference's 1D
     search range (3000 p^2 .. 30000 p^2 at the working pixel size p; outside
     it the reference itself fails) and by defocus band, agreement within the
@@ -372,11 +373,13 @@ def evaluate(torch, ctx, a, buf, thr):
         _, truth = make_batch(torch, m, a.size, a.pix, seed0=2000 + i0, out=buf, wide=True)
         lr, props = ctx.estimate_batch_learned(buf.data_ptr(), n, a.size, a.size, a.pix, proposals=True,
                                                conf_threshold=thr)
+        l0 = ctx.estimate_batch_learned(buf.data_ptr(), n, a.size, a.size, a.pix, conf_threshold=0.0)
         pr = ctx.estimate_batch(buf.data_ptr(), n, a.size, a.size, a.pix)
         for j in range(m):
             rows.append((truth[j][0], 0.5 * (lr[j]["df_min"] + lr[j]["df_max"]), 0.5 * (pr[j]["df_min"] + pr[j]["df_max"]),
-                         lr[j]["score"], pr[j]["score"], props[j]["fallback"], props[j]["confidence"]))
-    t, ml, mp, sl, sp, fb, conf = (np.array(c, np.float64) for c in zip(*rows))
+                         lr[j]["score"], pr[j]["score"], props[j]["fallback"], props[j]["confidence"],
+                         0.5 * (l0[j]["df_min"] + l0[j]["df_max"])))
+    t, ml, mp, sl, sp, fb, conf, m0 = (np.array(c, np.float64) for c in zip(*rows))
     dd = np.abs(ml - mp)
     inr = (t >= lo1) & (t <= hi1)
     step = np.maximum(0.2 * mp * 0.525 / 31.0, 20.0)
@@ -395,6 +398,8 @@ def evaluate(torch, ctx, a, buf, thr):
             "frac_within_plain_final_step_in_range": r(np.mean(dd[inr] <= step[inr])),
             "frac_learned_cc_ge_plain_where_apart": r(np.mean(sl[dis] >= sp[dis] - 1e-6)) if dis.any() else None,
             "fallback_frac": r(np.mean(fb)),
+            "conf_threshold": thr,
+            "frac_within_20A_without_confidence_fallback": r(np.mean(np.abs(m0 - mp) <= 20.0)),
             "median_confidence": round(float(np.median(conf)), 3),
             "median_abs_ddf_vs_truth_learned_A": round(float(np.median(np.abs(ml - t))), 1),
             "median_abs_ddf_vs_truth_plain_A": round(float(np.median(np.abs(mp - t))), 1),
