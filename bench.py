@@ -572,7 +572,9 @@ def cpu_baseline(imgs, pix, gpu, scan1d=None):
     for name, res in gpu.items():
         g = res[:t]
         d = lambda k: np.array([abs(r[k] - o[k]) for r, o in zip(g, refs)], np.float64)
-        dmin, dmax, daz, dcc = d("df_min"), d("df_max"), d("azimuth"), d("score")
+        dmin, dmax, dcc = d("df_min"), d("df_max"), d("score")
+        # the astigmatism azimuth is an axis: 180 degrees apart is the same one
+        daz = np.array([abs(math.remainder(r["azimuth"] - o["azimuth"], 180.0)) for r, o in zip(g, refs)])
         dmean = np.array([abs(0.5 * (r["df_min"] + r["df_max"]) - 0.5 * (o["df_min"] + o["df_max"]))
                           for r, o in zip(g, refs)])
         tol = (dmin <= 5.0) & (dmax <= 5.0) & (daz <= 0.5) & (dcc <= 1e-4)

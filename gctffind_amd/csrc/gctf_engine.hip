@@ -229,7 +229,7 @@ struct gctf_ctx {
 	std::string err;
 	int chunk = 4, tile_splits = 8, use_graph = 1;
 	int cpb = 8;          // candidates per 2D-evaluator workgroup (search blocking: 4, 8 or 16)
-	int list_queue = 1;   // tile passes of a list: 1 persistent workgroups on a work queue, 0 one item each
+	int list_queue = 1;   // tile row pass of a list: 1 persistent workgroups on a work queue, 0 one item each
 	int tune_gen = 0;     // bumped by gctf_autotune (list graphs read the tuned values)
 	float scan_lo = 0, scan_hi = 0, scan_step = 0;  // Do1D scan (gctf_set_search1d; 0: reference)
 	int gen = 0;  // bumped whenever a device buffer moves
@@ -1039,7 +1039,9 @@ int run_2d_program(gctf_ctx* c, const Search2DView& v, int m, int begin, const S
 	auto eval = [&](int maxc) {
 		const int cpb = c->cpb;
 		dim3 grid((maxc + cpb - 1) / cpb, kPixChunks, m);
-		if (cpb == 4)
+		if (cpb == 2)
+			PROF(c, "k_cc2d_eval", k_cc2d_eval<2><<<grid, 256, 0, s>>>(v.plans, v.ring_spec, v.kstride, v.st, v.part));
+		else if (cpb == 4)
 			PROF(c, "k_cc2d_eval", k_cc2d_eval<4><<<grid, 256, 0, s>>>(v.plans, v.ring_spec, v.kstride, v.st, v.part));
 		else if (cpb == 16)
 			PROF(c, "k_cc2d_eval", k_cc2d_eval<16><<<grid, 256, 0, s>>>(v.plans, v.ring_spec, v.kstride, v.st, v.part));
@@ -1090,9 +1092,11 @@ int launch_2d(gctf_ctx* c, int T, int i0, int m, int begin, float phase_range)
 
 // ---- learned path --------------------------------------------------------
 // Network outputs -> proposal (mean defocus A, astigmatism A, azimuth deg,
-// confidence) and the defocus window of the narrowed search.
-__global__ void k_proposal(const float* __restrict__ o, int n, float window, float* __restrict__ prop,
-	RefineArgs* __restrict__ refine, SearchState* __restrict__ st)
+// confidence) and the defocus window of the narrowed search.  A micrograph
+// whose confidence is below conf_thr goes to the plain search whatever the
+// narrowed search finds, so it sits the narrowed search out (skip).
+__global__ void k_proposal(const float* __restrict__ o, int n, float window, float conf_thr,
+	float* __restrict__ prop, RefineArgs* __restrict__ refine, SearchState* __restrict__ st)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
@@ -1112,6 +1116,7 @@ __global__ void k_proposal(const float* __restrict__ o, int n, float window, flo
 	r.phase[0] = 0.0f; r.phase[1] = 0.0f;
 	refine[i] = r;
 	st[i].conf = conf;
+	st[i].skip = conf < conf_thr ? 1 : 0;
 }
 
 // Hand the micrograph to the plain search when the proposal is not trusted
@@ -1340,8 +1345,8 @@ int gctf_synchronize(gctf_ctx* ctx)
 int gctf_tune(gctf_ctx* ctx, int chunk, int tile_splits, int use_graph, int search_block, int list_queue)
 {
 	if (!ctx) return GCTF_ERR_ARG;
-	if (search_block != 0 && search_block != 4 && search_block != 8 && search_block != 16)
-		return fail(ctx, GCTF_ERR_ARG, "search blocking: 4, 8 or 16 candidates per workgroup");
+	if (search_block != 0 && search_block != 2 && search_block != 4 && search_block != 8 && search_block != 16)
+		return fail(ctx, GCTF_ERR_ARG, "search blocking: 2, 4, 8 or 16 candidates per workgroup");
 	if (chunk > 0) ctx->chunk = chunk;
 	if (tile_splits > 0) { ctx->tile_splits = tile_splits; ctx->gen++; }
 	if (use_graph != 0) ctx->use_graph = use_graph > 0 ? 1 : 0;
@@ -1875,7 +1880,7 @@ int record_batch(gctf_ctx* c, const RescalePlan& rp, SearchPlan& sp, TileTables&
 			true, c->net_out.as<float>(), s));
 		HIPCHK(c, ne);
 		PROF(c, "k_proposal", k_proposal<<<(n + 63) / 64, 64, 0, s>>>(c->net_out.as<float>(), n, job.window,
-			c->props.as<float>(), c->refine.as<RefineArgs>(), c->states.as<SearchState>()));
+			job.conf_thr, c->props.as<float>(), c->refine.as<RefineArgs>(), c->states.as<SearchState>()));
 		CHK(fill_plans(c, 0, n, sp));
 		CHK(run_2d_program(c, view_2d(c, T, 0), n, NX_BEGIN_PROPOSAL, job.pg,
 			std::max(c->prm.ext_phase_range_deg, 0.0f)));
@@ -2133,6 +2138,10 @@ int gctf_estimate_batch_learned(gctf_ctx* ctx, const void* dev_imgs, int dtype, 
 		SearchState* st = ctx->states.as<SearchState>();
 		// one graph for every fallback pattern: which micrographs sit out is data
 		TuneScope tune(ctx, nx, ny, dtype, pix);
+		// few micrographs take part: smaller candidate blocks put more
+		// workgroups on the chip (the results do not depend on the blocking)
+		if (4 * nfb <= n) ctx->cpb = 2;
+		else if (2 * nfb <= n) ctx->cpb = std::min(ctx->cpb, 4);
 		char fkey[160];
 		snprintf(fkey, sizeof fkey, "fallback/%d/%d/%a/%a/%a/%d", n, ctx->prm.tile_size, sp->pix, range,
 			ctx->prm.ext_phase_deg, ctx->cpb);
@@ -2324,9 +2333,10 @@ int gctf_estimate_series(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, 
 // same pixel type whose row-pass output fits kListRtBytes; buffer offsets
 // restart at every group.  In a group, each rescale stage is one launch per
 // kernel class over all its micrographs (k_*_list), and the tile passes are
-// one launch each over the group's tile work list: persistent workgroups
-// pulling items from a work queue (k_tile_*_queue), or one workgroup per
-// item (k_tile_*_list) with gctf_tune's list mode 0.
+// one launch each over the group's tile work list.  The row pass runs as
+// persistent workgroups pulling items from a work queue (k_tile_rows_queue),
+// or one workgroup per item (k_tile_rows_list) with gctf_tune's list mode
+// -1; the column pass runs one workgroup per item (k_tile_cols_list).
 constexpr size_t kListRtBytes = 8ull << 30;
 
 struct ListGroup {
@@ -2336,7 +2346,7 @@ struct ListGroup {
 	int max_nr = 0;          // largest block-row count among jobs that need row sums
 	int max_np = 0, max_split = 0;
 	size_t pad = 0, rowsum = 0, sums = 0, rt = 0, part = 0, cbuf = 0;   // elements
-	int q = 0;               // first of its two work-queue counters
+	int q = 0;               // its row pass's work-queue counter
 };
 
 // One rescale stage's launch over the micrographs of one kernel class.
@@ -2451,22 +2461,21 @@ int launch_tile_list_t(gctf_ctx* c, const TileTables& tt, const TileJob* d_jobs,
 		PROF(c, "k_row_sums", k_row_sums_list<Tin><<<dim3(G.max_nr, G.nj), 256, 0, s>>>(jobs, c->rowsum.as<float>()));
 	PROF(c, "k_block_rows", k_block_rows_list<<<dim3((G.max_np + 255) / 256, G.nj), 256, 0, s>>>(jobs,
 		c->rowsum.as<float>(), c->bsums.as<float>()));
-	const int col_items = G.nj * G.max_split * NCG;
+	// The row pass's items differ in cost (rows of 8-bit, 16-bit and float
+	// micrographs, edge tiles), so it runs on a work queue; the column pass's
+	// items are uniform and run one per workgroup (a queue measured slower).
 	if (c->list_queue) {
-		int gr = 0, gc = 0;
+		int gr = 0;
 		CHK(resident_grid(c, k_tile_rows_queue<Tin, T, ROWS, RG>, (ROWS / 2) * (T / 8), G.items, &gr));
-		CHK(resident_grid(c, k_tile_cols_queue<T, CPW>, CPW * (T / 8), col_items, &gc));
 		PROF(c, "k_tile_rows_queue", k_tile_rows_queue<Tin, T, ROWS, RG><<<gr, (ROWS / 2) * (T / 8), 0, s>>>(jobs,
 			G.nj, G.items, qnext + G.q, c->bsums.as<float>(), tt.mask.as<float>(), tt.tw.as<float2>(),
 			c->rt.as<float2>()));
-		PROF(c, "k_tile_cols_queue", k_tile_cols_queue<T, CPW><<<gc, CPW * (T / 8), 0, s>>>(jobs, G.nj, G.max_split,
-			qnext + G.q + 1, c->rt.as<float2>(), tt.tw.as<float2>(), c->tpart.as<float>()));
 	} else {
 		PROF(c, "k_tile_rows", k_tile_rows_list<Tin, T, ROWS, RG><<<G.items, (ROWS / 2) * (T / 8), 0, s>>>(jobs, G.nj,
 			c->bsums.as<float>(), tt.mask.as<float>(), tt.tw.as<float2>(), c->rt.as<float2>()));
-		PROF(c, "k_tile_cols", k_tile_cols_list<T, CPW><<<dim3(NCG, G.max_split, G.nj), CPW * (T / 8),
-			0, s>>>(jobs, c->rt.as<float2>(), tt.tw.as<float2>(), c->tpart.as<float>()));
 	}
+	PROF(c, "k_tile_cols", k_tile_cols_list<T, CPW><<<dim3(NCG, G.max_split, G.nj), CPW * (T / 8),
+		0, s>>>(jobs, c->rt.as<float2>(), tt.tw.as<float2>(), c->tpart.as<float>()));
 	PROF(c, "k_tile_reduce", k_tile_reduce_list<<<dim3((T * CX + 255) / 256, G.nj), 256, 0, s>>>(jobs,
 		c->tpart.as<float>(), T * CX, c->raw.as<float>()));
 	HIPCHK(c, hipGetLastError());
@@ -2557,7 +2566,7 @@ int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result
 			|| (groups.back().rt + rt) * sizeof(float2) > kListRtBytes) {
 			ListGroup G;
 			G.j0 = k; G.f16 = f16;
-			G.q = 2 * (int)groups.size();
+			G.q = (int)groups.size();
 			groups.push_back(G);
 		}
 		ListGroup& G = groups.back();
@@ -2599,7 +2608,7 @@ int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result
 	CHK(ensure_buf(ctx, ctx->rt, sizeof(float2) * most.rt));
 	CHK(ensure_buf(ctx, ctx->tpart, sizeof(float) * most.part));
 	CHK(ensure_buf(ctx, ctx->jobs, sizeof(TileJob) * (size_t)n));
-	CHK(ensure_buf(ctx, ctx->qnext, sizeof(int) * 2 * groups.size()));
+	CHK(ensure_buf(ctx, ctx->qnext, sizeof(int) * groups.size()));
 	for (int k = 0; k < n; ++k) {
 		const int i = order[k];
 		jobs[(size_t)k].img = it[i].rp.active ? (const void*)(ctx->padc.as<float>() + pad_off[(size_t)i])
@@ -2680,7 +2689,7 @@ int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result
 	snprintf(tail, sizeof tail, "/%d/%d/%d", ctx->tile_splits, ctx->cpb, ctx->list_queue);
 	key += tail;
 	auto record = [&]() -> int {
-		const int nq = 2 * (int)groups.size();
+		const int nq = (int)groups.size();
 		k_queue_reset<<<(nq + 63) / 64, 64, 0, ctx->stream>>>(ctx->qnext.as<int>(), nq);
 		k_state_init<<<(n + 63) / 64, 64, 0, ctx->stream>>>(ctx->states.as<SearchState>(), n,
 			ctx->prm.ext_phase_deg, std::max(ctx->prm.ext_phase_range_deg, 0.0f));

@@ -826,34 +826,6 @@ __global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols_list(const TileJob*
 		blockIdx.x, blockIdx.y);
 }
 
-// Persistent form of k_tile_cols_list: item (j, s, cg) = (j nsplit_max + s)
-// ncg + cg for the jobs j, splits s < nsplit_max and column groups cg of
-// CPW columns; items of splits a job does not have are skipped.
-template <int T, int CPW>
-__global__ __launch_bounds__(CPW * (T / 8)) void k_tile_cols_queue(const TileJob* __restrict__ jobs, int njobs,
-	int nsplit_max, int* __restrict__ next, const float2* __restrict__ rt, const float2* __restrict__ tw,
-	float* __restrict__ part)
-{
-	constexpr int CX = T / 2 + 1;
-	constexpr int NCG = (CX + CPW - 1) / CPW;
-	const int items = njobs * nsplit_max * NCG;
-	__shared__ int s_item;
-	int item = blockIdx.x;
-	while (item < items) {
-		const int cg = item % NCG, r = item / NCG;
-		const int sp = r % nsplit_max, j = r / nsplit_max;
-		const TileJob& J = jobs[j];
-		if (sp < J.nsplit) {
-			const int ntiles = J.g.ntx * J.g.nty;
-			tile_cols_body<T, CPW>(rt + J.rt_off, ntiles, J.nsplit, 1.0f / (float)ntiles, tw, part + J.part_off, cg, sp);
-		}
-		__syncthreads();
-		if (threadIdx.x == 0) s_item = (int)gridDim.x + atomicAdd(next, 1);
-		__syncthreads();
-		item = s_item;
-	}
-}
-
 // Zero the work-queue counters of a list's persistent launches.
 __global__ void k_queue_reset(int* __restrict__ q, int n)
 {

@@ -189,8 +189,8 @@ GCTF_API int gctf_grid_search_batch(gctf_ctx* ctx, const void* dev_imgs, int dty
 
 /* Knobs of the batched path (0 keeps the current value): images per
  * spectrum chunk, tile-column splits, graph use (1 on, -1 off), 2D search
- * blocking (candidates per evaluator workgroup: 4, 8 or 16) and the tile
- * passes of gctf_estimate_list (1: persistent workgroups on a work queue,
+ * blocking (candidates per evaluator workgroup: 2, 4, 8 or 16) and the tile
+ * row pass of gctf_estimate_list (1: persistent workgroups on a work queue,
  * -1: one workgroup per item).  None of them changes results except the
  * tile splits (FP32 partial-sum grouping of the tile average). */
 GCTF_API int gctf_tune(gctf_ctx* ctx, int chunk, int tile_splits, int use_graph, int search_block,

@@ -45,7 +45,7 @@ def test_uneven_list_equals_single_batches():
     got = ctx.estimate_list(images)
     got2 = ctx.estimate_list(images)   # graph replay
     cg = gctf.Context()
-    cg.tune(list_queue=-1)             # tile passes with one workgroup per item
+    cg.tune(list_queue=-1)             # tile row pass with one workgroup per item
     got3 = cg.estimate_list(images)
     for i, im in enumerate(images):
         one = gctf.Context().estimate_batch(im["data"], 1, im["nx"], im["ny"], im["pix"], dtype=im["dtype"])[0]
@@ -56,7 +56,7 @@ def test_uneven_list_equals_single_batches():
 
 
 def test_search_blocking_keeps_results():
-    """4, 8 or 16 candidates per 2D-evaluator workgroup: identical results
+    """2, 4, 8 or 16 candidates per 2D-evaluator workgroup: identical results
     (each candidate's sums are formed on their own)."""
     gctf = _setup()
     from gctffind_amd import synth
@@ -67,11 +67,11 @@ def test_search_blocking_keeps_results():
                          az_deg=-50.0 + 40 * i, device="cuda", out=imgs[i])
     torch.cuda.synchronize()
     res = {}
-    for cb in (4, 8, 16):
+    for cb in (2, 4, 8, 16):
         ctx = gctf.Context()
         ctx.tune(search_block=cb)
         res[cb] = ctx.estimate_batch(imgs.data_ptr(), n, 1024, 1024, 1.2)
-    for cb in (4, 16):
+    for cb in (2, 4, 16):
         for i in range(n):
             for k in ("df_min", "df_max", "azimuth", "score", "ctf_res"):
                 assert res[cb][i][k] == res[8][i][k], (cb, i, k)
