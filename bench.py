@@ -27,7 +27,9 @@ evaluation set.  Extra objects:
                 micrograph under config.parity_vs_oracle
 Further legs in the same run (each skippable): the 256-micrograph BF16 grid
 search (config.grid_leg), the uneven list of 64 micrographs
-(config.uneven_list) and a 41-micrograph tilt series (config.series_leg).
+(config.uneven_list), a 41-micrograph tilt series (config.series_leg) and
+the GCtfFind command line end to end on a folder of MRC files
+(config.files_leg).
 """
 import argparse
 import json
@@ -126,6 +128,9 @@ def parse():
     ap.add_argument("--uneven-leg", type=int, default=64, metavar="N",
                     help="also time gctf_estimate_list on N uneven micrographs (half 4096^2 f32, a quarter "
                          "5760x4092 f16, a quarter 11520x8184 f16, 0.5-1.1 A); reported under config.uneven_list")
+    ap.add_argument("--files-leg", type=int, default=40, metavar="N",
+                    help="also time the GCtfFind command line end to end on N 4096^2 float32 MRC files "
+                         "(0: skip); reported under config.files_leg")
     ap.add_argument("--no-plain-leg", action="store_true",
                     help="learned mode: do not time the plain path beside it (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -294,6 +299,10 @@ def main():
         log(f"tilt-series leg: {a.series_leg} micrographs")
         out["config"]["series_leg"] = series_leg(torch, gctf, a.series_leg, a.steps, a.warmup)
         torch.cuda.empty_cache()
+    if a.files_leg:
+        log(f"files leg: GCtfFind on {a.files_leg} MRC files")
+        out["config"]["files_leg"] = files_leg(torch, a, a.files_leg)
+        torch.cuda.empty_cache()
     if not a.no_cpu_baseline:
         log(f"CPU oracle on {len(host)} micrographs")
         gpu = {"plain": pres if learned else res}
@@ -321,6 +330,89 @@ def timed(torch, run, n, steps, warmup):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return n * steps / dt, dt * 1000.0 / steps, res
+
+
+def write_mrc_f32(path, img, pix):
+    """One float32 section as an MRC2014 file (little endian, no extended
+    header)."""
+    import struct
+    ny, nx = img.shape
+    h = bytearray(1024)
+    struct.pack_into("<10i", h, 0, nx, ny, 1, 2, 0, 0, 0, nx, ny, 1)
+    struct.pack_into("<6f", h, 40, nx * pix, ny * pix, pix, 90.0, 90.0, 90.0)
+    struct.pack_into("<3i", h, 64, 1, 2, 3)
+    h[208:212] = b"MAP "
+    h[212:214] = b"\x44\x44"
+    with open(path, "wb") as f:
+        f.write(bytes(h))
+        f.write(img.astype("<f4", copy=False).tobytes())
+
+
+def files_leg(torch, a, n, n_small=8):
+    """End to end on files: the GCtfFind command line over a serial folder of
+    n MRC files (float32 4096^2 micrographs at 1.0 A, seeds 3000..: one
+    collection session, mean defocus uniform 13500-16500 A, astigmatism
+    0-1000 A, B 100, SNR 0.1), writing the spectra (-OutMrc) and the CTF
+    table (-OutCtf).  The program reads the next files on its loader thread while
+    the GPU estimates the current one, and its writer thread saves earlier
+    spectra (the CLoadImages / CAsyncSaveImages overlap).  Each micrograph
+    goes through the per-micrograph API from host memory; the first gets the
+    full search, the others Refine around the best-scoring one so far
+    (CFindSeriesCtfs).  Two runs of the program, on the first n_small files
+    and on all n, so that the per-run start-up (process, GPU context, plans)
+    cancels: value = (n - n_small) / (t_n - t_small).  The files are in the
+    page cache (just written)."""
+    import shutil
+    import subprocess
+    import tempfile
+    import numpy as np
+    cli = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gctffind_amd", "GCtfFind")
+    if not os.path.exists(cli):
+        raise RuntimeError("gctffind_amd/GCtfFind is not built (python -m gctffind_amd.build)")
+    n_small = max(1, min(n_small, n - 1))
+    d = tempfile.mkdtemp(prefix="gctf_files_")
+    try:
+        from gctffind_amd import synth
+        buf = torch.empty((a.size, a.size), dtype=torch.float32, device="cuda:0")
+        truth = []
+        for i in range(n):
+            rng = np.random.default_rng(3000 + i)
+            dfm, ast = float(rng.uniform(13500.0, 16500.0)), float(rng.uniform(0.0, 1000.0))
+            synth.micrograph(a.size, a.size, pix=a.pix, seed=3000 + i, df1=dfm + ast / 2, df2=dfm - ast / 2,
+                             az_deg=float(rng.uniform(-90.0, 90.0)), bfac=100.0, snr=0.1, device="cuda:0", out=buf)
+            truth.append(dfm)
+            write_mrc_f32(os.path.join(d, f"mic_{i + 1:03d}.mrc"), buf.cpu().numpy(), a.pix)
+        del buf
+        runs = {}
+        for m in (n_small, n):
+            sub = os.path.join(d, f"run{m}")
+            os.mkdir(sub)
+            for i in range(m):
+                os.symlink(os.path.join(d, f"mic_{i + 1:03d}.mrc"), os.path.join(sub, f"mic_{i + 1:03d}.mrc"))
+            cmd = [cli, "-InMrc", os.path.join(sub, "mic_"), "-InSuffix", ".mrc", "-Serial", "1",
+                   "-OutMrc", os.path.join(sub, "spec.mrc"), "-OutCtf", os.path.join(sub, "ctf.txt"),
+                   "-PixSize", str(a.pix)]
+            t0 = time.perf_counter()
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+            dt = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError(f"GCtfFind exited {r.returncode}: {r.stderr[-2000:]}")
+            rows = [ln.split() for ln in open(os.path.join(sub, "ctf.txt")) if not ln.startswith("#")]
+            dfm = np.array([0.5 * (float(x[2]) + float(x[3])) for x in rows])
+            runs[m] = (dt, dfm)
+        (t1, _), (t2, dfm) = runs[n_small], runs[n]
+        err = np.abs(dfm - np.array(truth))
+        mb = a.size * a.size * 4 / 1e6
+        return {"micrographs": n, "value": round((n - n_small) / (t2 - t1), 2), "unit": "micrographs/s",
+                "wall_s": {str(n_small): round(t1, 3), str(n): round(t2, 3)},
+                "file": f"{a.size}x{a.size} float32 MRC at {a.pix} A, {mb:.1f} MB",
+                "input_MB_per_s": round((n - n_small) * mb / (t2 - t1), 1),
+                "program": "gctffind_amd/GCtfFind -Serial 1 -OutMrc -OutCtf: first micrograph full search, "
+                           "the rest Refine around the best-scoring one",
+                "session": "mean defocus uniform 13500-16500 A, astigmatism 0-1000 A",
+                "median_abs_ddf_vs_truth_A": round(float(np.median(err)), 1)}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 def uneven_list(torch, gctf, ctx, n, steps, warmup):

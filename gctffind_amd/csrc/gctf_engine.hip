@@ -1048,8 +1048,8 @@ int run_2d_program(gctf_ctx* c, const Search2DView& v, int m, int begin, const S
 		else
 			PROF(c, "k_cc2d_eval", k_cc2d_eval<8><<<grid, 256, 0, s>>>(v.plans, v.ring_spec, v.kstride, v.st, v.part));
 	};
-	auto step = [&](int next) {
-		PROF(c, "k_s2d_step", k_s2d_step<<<m, 256, 0, s>>>(v.st, v.part, v.ring_ss, next, v.refine, pg));
+	auto step = [&](int next, int next2 = -1) {
+		PROF(c, "k_s2d_step", k_s2d_step<<<m, 256, 0, s>>>(v.st, v.part, v.ring_ss, next, next2, v.refine, pg));
 	};
 	int maxph = 61;
 	if (phase_range > 61.0f) maxph = std::min(kMaxCands, (int)phase_range + 3);
@@ -1066,9 +1066,14 @@ int run_2d_program(gctf_ctx* c, const Search2DView& v, int m, int begin, const S
 		eval(pg.n_angl + 1);
 		step(NX_DFMEAN);
 		eval(pg.n_df + 1);
-		step(NX_PHASE);
-		if (phase_range > 0.0f) eval(maxph);
-		step(i < pg.iters - 1 ? NX_ASTMAG : last);
+		const int after = i < pg.iters - 1 ? NX_ASTMAG : last;
+		if (phase_range > 0.0f) {
+			step(NX_PHASE);
+			eval(maxph);
+			step(after);
+		} else {
+			step(NX_PHASE, after);   // the phase stage prepares nothing
+		}
 	}
 	if (last == NX_FINE) {
 		eval(pg.fine_n + 1);

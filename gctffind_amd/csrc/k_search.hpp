@@ -575,11 +575,65 @@ struct RefineArgs {
 	float dfmean[2], ast[2], ang[2], phase[2];
 };
 
+// Prepare stage `next` of state s (prev_kind: the kind of the stage just
+// consumed) and, at the end of the search, write the results.
+__device__ void advance_state(SearchState& s, int prev_kind, int next)
+{
+	if (!s.done) {
+		switch (next) {
+		case NX_ASTIG:
+			prep_astig(s);
+			break;
+		case NX_ASTMAG:
+			if (prev_kind == ST_ASTIG || s.iter == 0) {
+				// after mFindAstig (CFindDefocus2D.cpp:158-166)
+				if (s.oldcc > s.ccmax) { s.done = 1; break; }
+				s.phase = s.phrange[0];
+				s.tot_df = s.dfrange[1] - s.dfrange[0];
+				s.tot_ast = s.astrange[1] - s.astrange[0];
+				s.tot_ang = s.angrange[1] - s.angrange[0];
+			}
+			s.iter += 1;
+			prep_line(s, ST_ASTMAG, s.ast, s.astrange, iter_scale(s, s.iter) * s.tot_ast, s.pg.n_mag, 0.001f);
+			break;
+		case NX_ASTANG:
+			// CFindDefocus2D.cpp:171-173
+			if (s.oldcc > s.ccmax) { s.done = 1; break; }
+			s.ccmax = s.oldcc;
+			prep_line(s, ST_ASTANG, s.ang, s.angrange, iter_scale(s, s.iter) * s.tot_ang, s.pg.n_angl, 1.0f);
+			break;
+		case NX_DFMEAN:
+			prep_dfmean(s, iter_scale(s, s.iter) * s.tot_df);
+			break;
+		case NX_PHASE:
+			prep_phase(s, iter_scale(s, s.iter) * s.phrange[1]);
+			break;
+		case NX_FINE:
+			prep_fine(s);
+			break;
+		case NX_FINISH:
+			prep_none(s);
+			break;
+		default: break;
+		}
+	}
+	if (next == NX_FINISH || s.done) {
+		// CFindDefocus2D::GetDfMin/GetDfMax/... (CFindDefocus2D.cpp:38-67)
+		s.df_min = s.dfmean * (1.0f - s.ast);
+		s.df_max = s.dfmean * (1.0f + s.ast);
+		s.ast_ang = s.ang;
+		s.ext_phase = s.phase;
+		s.score = s.ccmax;
+		if (s.done) prep_none(s);
+	}
+}
+
 // One workgroup (256 threads) per micrograph: consume the CCs of the stage
 // just evaluated (first strict maximum, `<= continue` order), update the
-// state as the reference method does, and prepare the next stage.
+// state as the reference method does, and prepare the next stage (then
+// next2's, when next2 >= 0 and stage `next` evaluates nothing).
 __global__ __launch_bounds__(256) void k_s2d_step(SearchState* __restrict__ st,
-	const double2* __restrict__ part, const double* __restrict__ ring_ss, int next,
+	const double2* __restrict__ part, const double* __restrict__ ring_ss, int next, int next2,
 	const RefineArgs* __restrict__ refine, SearchProgram prog)
 {
 	__shared__ float s_cc[256];
@@ -699,53 +753,10 @@ __global__ __launch_bounds__(256) void k_s2d_step(SearchState* __restrict__ st,
 		default: break;
 		}
 	}
-	if (!s.done) {
-		switch (next) {
-		case NX_ASTIG:
-			prep_astig(s);
-			break;
-		case NX_ASTMAG:
-			if (s0.st_kind == ST_ASTIG || s.iter == 0) {
-				// after mFindAstig (CFindDefocus2D.cpp:158-166)
-				if (s.oldcc > s.ccmax) { s.done = 1; break; }
-				s.phase = s.phrange[0];
-				s.tot_df = s.dfrange[1] - s.dfrange[0];
-				s.tot_ast = s.astrange[1] - s.astrange[0];
-				s.tot_ang = s.angrange[1] - s.angrange[0];
-			}
-			s.iter += 1;
-			prep_line(s, ST_ASTMAG, s.ast, s.astrange, iter_scale(s, s.iter) * s.tot_ast, s.pg.n_mag, 0.001f);
-			break;
-		case NX_ASTANG:
-			// CFindDefocus2D.cpp:171-173
-			if (s.oldcc > s.ccmax) { s.done = 1; break; }
-			s.ccmax = s.oldcc;
-			prep_line(s, ST_ASTANG, s.ang, s.angrange, iter_scale(s, s.iter) * s.tot_ang, s.pg.n_angl, 1.0f);
-			break;
-		case NX_DFMEAN:
-			prep_dfmean(s, iter_scale(s, s.iter) * s.tot_df);
-			break;
-		case NX_PHASE:
-			prep_phase(s, iter_scale(s, s.iter) * s.phrange[1]);
-			break;
-		case NX_FINE:
-			prep_fine(s);
-			break;
-		case NX_FINISH:
-			prep_none(s);
-			break;
-		default: break;
-		}
-	}
-	if (next == NX_FINISH || s.done) {
-		// CFindDefocus2D::GetDfMin/GetDfMax/... (CFindDefocus2D.cpp:38-67)
-		s.df_min = s.dfmean * (1.0f - s.ast);
-		s.df_max = s.dfmean * (1.0f + s.ast);
-		s.ast_ang = s.ang;
-		s.ext_phase = s.phase;
-		s.score = s.ccmax;
-		if (s.done) prep_none(s);
-	}
+	advance_state(s, s0.st_kind, next);
+	// a second transition with no stage evaluated in between (the phase
+	// stage of a search without phase range prepares nothing)
+	if (next2 >= 0 && !s.done) advance_state(s, s.st_kind, next2);
 	*S = s;
 }
 
