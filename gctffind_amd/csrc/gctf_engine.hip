@@ -1347,6 +1347,25 @@ int gctf_synchronize(gctf_ctx* ctx)
 	return GCTF_OK;
 }
 
+int gctf_host_alloc(size_t bytes, void** out)
+{
+	if (!out) return GCTF_ERR_ARG;
+	*out = nullptr;
+	if (bytes == 0) return GCTF_ERR_ARG;
+	void* p = nullptr;
+	if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) {
+		(void)hipGetLastError();
+		return GCTF_ERR_NOMEM;
+	}
+	*out = p;
+	return GCTF_OK;
+}
+
+void gctf_host_free(void* p)
+{
+	if (p) (void)hipHostFree(p);
+}
+
 int gctf_tune(gctf_ctx* ctx, int chunk, int tile_splits, int use_graph, int search_block, int list_queue)
 {
 	if (!ctx) return GCTF_ERR_ARG;

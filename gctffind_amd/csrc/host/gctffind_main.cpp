@@ -214,20 +214,27 @@ void save_imod(const Options& o, const std::vector<Package>& p, bool tomo)
 	fclose(fp);
 }
 
-// CLoadImages::ThreadMain (CLoadImages.cpp:64-137): micrographs are read on a
-// thread of their own, in processing order, while the GPU works on earlier
-// ones.  At most `ahead` read micrographs wait in memory.
+// CLoadImages::ThreadMain (CLoadImages.cpp:64-137): micrographs are read
+// ahead of the GPU, in processing order, while it works on earlier ones.
+// `readers` threads read different files at once, each straight into a
+// buffer of a pool of page-locked host memory (gctf_host_alloc; ordinary
+// memory if that fails).  The consumer hands a buffer back once the GPU has
+// copied its image, so at most `ahead` micrographs wait and no file costs a
+// fresh allocation.
 class ImageLoader {
 public:
 	struct Item {
 		bool ok = false;
 		int nx = 0, ny = 0, mode = 0;
-		std::vector<float> px;
+		float* px = nullptr;   // nx*ny floats, the loader's until give_back
+		int buf = -1;
 	};
-	ImageLoader(const std::vector<Package>& pk, const std::vector<int>& order, size_t ahead)
-		: pk_(pk), order_(order), ahead_(ahead), items_(order.size()), ready_(order.size(), 0)
+	ImageLoader(const std::vector<Package>& pk, const std::vector<int>& order, size_t ahead, int readers)
+		: pk_(pk), order_(order), ahead_(std::max<size_t>(ahead, 1)), items_(order.size()),
+		  ready_(order.size(), 0), bufs_(ahead_ + 1)
 	{
-		th_ = std::thread([this] { run(); });
+		for (int i = 0; i < (int)bufs_.size(); ++i) free_.push_back(i);
+		for (int t = 0; t < std::max(readers, 1); ++t) th_.emplace_back([this] { run(); });
 	}
 	~ImageLoader()
 	{
@@ -236,48 +243,89 @@ public:
 			stop_ = true;
 		}
 		cv_.notify_all();
-		th_.join();
+		for (auto& t : th_) t.join();
+		for (auto& b : bufs_) release(b);
 	}
 	// The k-th micrograph of the processing order; waits until it is read.
 	Item take(size_t k)
 	{
 		std::unique_lock<std::mutex> l(m_);
 		cv_.wait(l, [&] { return ready_[k] != 0; });
-		Item it = std::move(items_[k]);
-		taken_ = k + 1;
-		l.unlock();
+		return items_[k];
+	}
+	// Returns the buffer of the k-th micrograph (taken with take(k)); no
+	// copy from it may be pending.
+	void give_back(size_t k, const Item& it)
+	{
+		{
+			std::lock_guard<std::mutex> l(m_);
+			if (it.buf >= 0) free_.push_back(it.buf);
+			taken_ = k + 1;
+		}
 		cv_.notify_all();
-		return it;
 	}
 
 private:
+	struct Buf {
+		float* p = nullptr;
+		size_t cap = 0;
+		bool pinned = false;
+	};
+	static void release(Buf& b)
+	{
+		if (b.p) {
+			if (b.pinned) gctf_host_free(b.p);
+			else std::free(b.p);
+		}
+		b = Buf();
+	}
+	static bool reserve(Buf& b, size_t n)
+	{
+		if (b.cap >= n) return true;
+		release(b);
+		void* p = nullptr;
+		if (gctf_host_alloc(n * sizeof(float), &p) == GCTF_OK) {
+			b.pinned = true;
+		} else {
+			p = std::malloc(n * sizeof(float));
+			if (!p) return false;
+		}
+		b.p = static_cast<float*>(p);
+		b.cap = n;
+		return true;
+	}
 	void run()
 	{
-		for (size_t k = 0; k < order_.size(); ++k) {
+		for (;;) {
+			size_t k;
+			int bi;
 			{
 				std::unique_lock<std::mutex> l(m_);
-				cv_.wait(l, [&] { return stop_ || k < taken_ + ahead_; });
-				if (stop_) return;
+				cv_.wait(l, [&] {
+					return stop_ || next_ >= order_.size() || (next_ < taken_ + ahead_ && !free_.empty());
+				});
+				if (stop_ || next_ >= order_.size()) return;
+				k = next_++;
+				bi = free_.back();
+				free_.pop_back();
 			}
 			const Package& p = pk_[order_[k]];
 			Item it;
+			it.buf = bi;
 			MrcReader r;
-			// a failed read (including a section buffer that cannot be
-			// allocated) is reported to the consumer as a read error; an
-			// exception must not end this thread, which would terminate
-			// the program
-			try {
-				it.ok = r.Open(p.file) && r.ReadSection(p.section, it.px);
-			} catch (const std::exception&) {
-				it.ok = false;
-				it.px.clear();
+			// a failed read, including a buffer that cannot be allocated, is
+			// reported to the consumer as a read error
+			if (r.Open(p.file)) {
+				it.nx = r.Header().nx;
+				it.ny = r.Header().ny;
+				it.mode = r.Header().mode;
+				Buf& b = bufs_[bi];
+				it.ok = reserve(b, (size_t)it.nx * it.ny) && r.ReadSection(p.section, b.p);
+				it.px = b.p;
 			}
-			it.nx = r.Header().nx;
-			it.ny = r.Header().ny;
-			it.mode = r.Header().mode;
 			{
 				std::lock_guard<std::mutex> l(m_);
-				items_[k] = std::move(it);
+				items_[k] = it;
 				ready_[k] = 1;
 			}
 			cv_.notify_all();
@@ -288,11 +336,13 @@ private:
 	const size_t ahead_;
 	std::vector<Item> items_;
 	std::vector<char> ready_;
-	size_t taken_ = 0;
+	std::vector<Buf> bufs_;
+	std::vector<int> free_;
+	size_t next_ = 0, taken_ = 0;
 	bool stop_ = false;
 	std::mutex m_;
 	std::condition_variable cv_;
-	std::thread th_;
+	std::vector<std::thread> th_;
 };
 
 // CAsyncSaveImages (CAsyncSaveImages.cpp:31-110): full spectra are written on
@@ -487,7 +537,7 @@ int main(int argc, char** argv)
 
 	std::unique_ptr<SpectrumWriter> writer;
 	if (!o.out_mrc.empty()) writer = std::make_unique<SpectrumWriter>();
-	ImageLoader loader(pk, order, 2);
+	ImageLoader loader(pk, order, 4, 4);
 	CRescaleImage rescale;
 	CFindCtf2D find;
 	int ref = -1;
@@ -500,13 +550,12 @@ int main(int argc, char** argv)
 			fprintf(stderr, "Error: cannot read %s (section %d, mode %d)\n", p.file.c_str(), p.section, im.mode);
 			return 1;
 		}
-		std::vector<float>& img = im.px;
 		auto t0 = std::chrono::steady_clock::now();
 		int size[2] = {im.nx, im.ny};
 		try {
 			// CFindSeriesCtfs::mRescaleImage + mSetupFindCtf (:108-131)
 			rescale.Setup(size, o.pix);
-			rescale.DoIt(img.data());
+			rescale.DoIt(im.px);
 			CCTFTheory theory;
 			theory.Setup(o.kv, o.cs, o.ac, rescale.m_fPixSizeN, 100.0f, o.ext_phase[0] * 0.017453f);
 			find.Setup1(&theory);
@@ -548,6 +597,12 @@ int main(int argc, char** argv)
 			if (pk[ref].score < p.score) ref = ii;
 			if (p.score <= 0) { full(); take(); }
 		}
+		// the image's copy to the GPU is done: its buffer can take the next file
+		if (gctf_synchronize(SharedContext()) != GCTF_OK) {
+			fprintf(stderr, "Error: %s\n", gctf_error_string(SharedContext()));
+			return 1;
+		}
+		loader.give_back(k, im);
 		// mDisplay (:215-239)
 		printf("%s: %5d prcoessed, %5d left\n   Index  dfmin     dfmax    azimuth  phase   score\n"
 		       "   %4d  %8.2f  %8.2f  %6.2f %6.2f %9.5f\n\n",

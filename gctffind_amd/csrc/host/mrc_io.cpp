@@ -86,13 +86,25 @@ void MrcReader::Close(void)
 bool MrcReader::ReadSection(int iz, std::vector<float>& out)
 {
 	if (!m_fp || iz < 0 || iz >= m_h.nz) return false;
+	out.resize((size_t)m_h.nx * m_h.ny);
+	return ReadSection(iz, out.data());
+}
+
+// The section's bytes are read into the tail of `out` and converted to
+// float32 front to back in place: value i (bpp bytes at byte offset
+// n (4 - bpp) + bpp i) is read before float i (bytes [4 i, 4 i + 4)) is
+// written, and float i overwrites only values < i + 1, since
+// 4 (i + 1) <= n (4 - bpp) + bpp (i + 1) for i < n.
+bool MrcReader::ReadSection(int iz, float* out)
+{
+	if (!m_fp || !out || iz < 0 || iz >= m_h.nz) return false;
 	const int bpp = MrcModeBytes(m_h.mode);
+	if (bpp == 0) return false;
 	const size_t n = (size_t)m_h.nx * m_h.ny;
 	const long long off = 1024LL + m_h.nsymbt + (long long)iz * (long long)n * bpp;
 	if (fseeko(m_fp, off, SEEK_SET) != 0) return false;
-	std::vector<unsigned char> raw(n * bpp);
-	if (fread(raw.data(), 1, raw.size(), m_fp) != raw.size()) return false;
-	out.resize(n);
+	unsigned char* raw = reinterpret_cast<unsigned char*>(out) + n * (4 - bpp);
+	if (fread(raw, 1, n * bpp, m_fp) != n * bpp) return false;
 	const bool sw = m_h.swapped;
 	switch (m_h.mode) {
 	case 0: case 5:
@@ -100,31 +112,32 @@ bool MrcReader::ReadSection(int iz, std::vector<float>& out)
 		break;
 	case 1:
 		for (size_t i = 0; i < n; ++i) {
-			uint16_t v; memcpy(&v, &raw[2 * i], 2);
+			uint16_t v; memcpy(&v, raw + 2 * i, 2);
 			if (sw) v = bswap16(v);
 			out[i] = (float)(int16_t)v;
 		}
 		break;
 	case 6:
 		for (size_t i = 0; i < n; ++i) {
-			uint16_t v; memcpy(&v, &raw[2 * i], 2);
+			uint16_t v; memcpy(&v, raw + 2 * i, 2);
 			if (sw) v = bswap16(v);
 			out[i] = (float)v;
 		}
 		break;
 	case 12:
 		for (size_t i = 0; i < n; ++i) {
-			uint16_t v; memcpy(&v, &raw[2 * i], 2);
+			uint16_t v; memcpy(&v, raw + 2 * i, 2);
 			if (sw) v = bswap16(v);
 			out[i] = half_to_float(v);
 		}
 		break;
 	case 2:
-		for (size_t i = 0; i < n; ++i) {
-			uint32_t v; memcpy(&v, &raw[4 * i], 4);
-			if (sw) v = bswap32(v);
-			memcpy(&out[i], &v, 4);
-		}
+		if (sw)
+			for (size_t i = 0; i < n; ++i) {
+				uint32_t v; memcpy(&v, raw + 4 * i, 4);
+				v = bswap32(v);
+				memcpy(&out[i], &v, 4);
+			}
 		break;
 	default:
 		return false;

@@ -27,6 +27,8 @@ public:
 	// section iz converted to float32 (nx*ny values); false on I/O error or
 	// an unsupported mode
 	bool ReadSection(int iz, std::vector<float>& out);
+	// the same into out, which holds at least nx*ny floats
+	bool ReadSection(int iz, float* out);
 	// tilt of section iz from the extended header; false when there is none
 	bool Tilt(int iz, float* tilt);
 	~MrcReader() { Close(); }

@@ -22,6 +22,7 @@ DTYPE_F32, DTYPE_F16 = 0, 1
 
 EXPORTS = [
     "gctf_create", "gctf_destroy", "gctf_error_string", "gctf_set_stream", "gctf_synchronize",
+    "gctf_host_alloc", "gctf_host_free",
     "gctf_rescale_setup", "gctf_rescale", "gctf_scaled_image", "gctf_setup1", "gctf_setup2",
     "gctf_set_phase", "gctf_set_search1d", "gctf_gen_half_spectrum", "gctf_do1d", "gctf_refine1d", "gctf_do2d",
     "gctf_do2d_batch_static",
@@ -113,6 +114,8 @@ def load():
         "gctf_error_string": (ctypes.c_char_p, [vp]),
         "gctf_set_stream": (ci, [vp, vp]),
         "gctf_synchronize": (ci, [vp]),
+        "gctf_host_alloc": (ci, [ctypes.c_size_t, ctypes.POINTER(vp)]),
+        "gctf_host_free": (None, [vp]),
         "gctf_rescale_setup": (ci, [vp, ci, ci, cf, fp, fp]),
         "gctf_rescale": (ci, [vp, fp, ci]),
         "gctf_scaled_image": (ci, [vp, ctypes.POINTER(vp)]),

@@ -16,6 +16,8 @@
 #ifndef GCTF_HIP_H
 #define GCTF_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -64,6 +66,13 @@ GCTF_API const char* gctf_error_string(const gctf_ctx* ctx);
 GCTF_API int gctf_set_stream(gctf_ctx* ctx, void* hip_stream);
 /* Wait for all work of the context. */
 GCTF_API int gctf_synchronize(gctf_ctx* ctx);
+/* Page-locked host memory (no reference counterpart) for callers that stage
+ * micrographs on the host, e.g. the command line's loader threads: copies
+ * from it to the device run at the full PCIe rate and without an extra
+ * host copy.  GCTF_ERR_NOMEM when the allocation fails (*out null).  Free
+ * with gctf_host_free once no copy from it is pending. */
+GCTF_API int gctf_host_alloc(size_t bytes, void** out);
+GCTF_API void gctf_host_free(void* p);
 
 /* ---- CRescaleImage (CFindCTFInc.h:422-442) ---------------------------- */
 /* CRescaleImage::Setup(int* piRawSize, float fRawPixSize) (:428).

@@ -10,6 +10,7 @@
 // few milliseconds so the loader and writer threads run beside it.
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -48,6 +49,17 @@ int gctf_create(const gctf_params* params, int device, gctf_ctx** out)
 void gctf_destroy(gctf_ctx* ctx) { delete ctx; }
 
 const char* gctf_error_string(const gctf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int gctf_synchronize(gctf_ctx* ctx) { return ctx ? GCTF_OK : GCTF_ERR_ARG; }
+
+int gctf_host_alloc(size_t bytes, void** out)
+{
+	if (!out) return GCTF_ERR_ARG;
+	*out = bytes ? std::malloc(bytes) : nullptr;
+	return *out ? GCTF_OK : (bytes ? GCTF_ERR_NOMEM : GCTF_ERR_ARG);
+}
+
+void gctf_host_free(void* p) { std::free(p); }
 
 int gctf_rescale_setup(gctf_ctx* ctx, int nx, int ny, float raw_pix, int* new_size, float* new_pix)
 {
