@@ -542,6 +542,7 @@ int main(int argc, char** argv)
 	CFindCtf2D find;
 	int ref = -1;
 	int done = 0;
+	auto t_first = std::chrono::steady_clock::now();
 	for (size_t k = 0; k < order.size(); ++k) {
 		const int ii = order[k];
 		Package& p = pk[ii];
@@ -635,11 +636,19 @@ int main(int argc, char** argv)
 		double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 		printf("Processing time: %.2f (s)\n", sec);
 		done += 1;
+		if (done == 1) t_first = std::chrono::steady_clock::now();
 		if (done % 21 == 0) { save_ctf(o, pk); save_imod(o, pk, tomo); }
 	}
 	if (writer) writer->finish();
 	save_ctf(o, pk);
 	save_imod(o, pk, tomo);
+	if (done > 1) {
+		// the micrographs after the first, end to end (reads, GPU, spectra
+		// and tables written), without program start-up and the first
+		// micrograph's plan set-up
+		const double st = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_first).count();
+		printf("Steady state: %d micrographs in %.4f (s) after the first\n", done - 1, st);
+	}
 	gctf_destroy(SharedContext());
 	double tot = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
 	printf("Total time:  %.3f (s)\n\n", tot);

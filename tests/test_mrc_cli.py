@@ -54,12 +54,18 @@ def test_read_modes(tmp_path, mode, big):
         assert s["mean"] == pytest.approx(ref[z].mean(), rel=1e-6, abs=1e-6)
         assert s["first"] == pytest.approx(ref[z, 0, 0], rel=1e-6) and s["last"] == pytest.approx(ref[z, -1, -1], rel=1e-6)
         assert s["has_tilt"] == 1 and s["tilt"] == pytest.approx([-3.0, 0.0, 3.0][z])
+    # every value: the sections converted to float32 (read in place into the
+    # output buffer) and written back as a float32 stack
+    out = str(tmp_path / "f32.mrc")
+    r = run("--mrcinfo", f, out)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert np.array_equal(read_mrc(out), ref.astype(np.float32))
 
 
 def test_float32_writer_round_trip(tmp_path):
     d = np.random.default_rng(1).standard_normal((2, 16, 24)).astype(np.float32)
     src, dst = str(tmp_path / "in.mrc"), str(tmp_path / "out.mrc")
-    write_mrc(src, d, mode=1 if False else 2)
+    write_mrc(src, d, mode=2)
     r = run("--mrcinfo", src, dst)
     assert r.returncode == 0
     back = read_mrc(dst)
