@@ -93,17 +93,49 @@ void host_fft(std::vector<std::complex<double>>& a, bool inv)
 // Device tables of one transform length.
 struct FftDev {
 	int n = 0, m = 0;
+	std::vector<unsigned char> rad;   // mixed-radix passes (empty: power of two or Bluestein)
 	DevBuf tw, chirp, bhat;
 	Fft1dPlan plan() const
 	{
-		Fft1dPlan p;
+		Fft1dPlan p{};
 		p.n = n; p.m = m;
 		p.tw = tw.as<float2>();
 		p.chirp = chirp.p ? chirp.as<float2>() : nullptr;
 		p.bhat = bhat.p ? bhat.as<float2>() : nullptr;
+		p.nrad = (int)rad.size();
+		for (size_t i = 0; i < rad.size(); ++i) p.rad[i] = rad[i];
 		return p;
 	}
 };
+
+// n = 2^a 3^b 5^c
+bool smooth235(int n)
+{
+	if (n < 1) return false;
+	while (n % 2 == 0) n /= 2;
+	while (n % 3 == 0) n /= 3;
+	while (n % 5 == 0) n /= 5;
+	return n == 1;
+}
+
+// The length a transform of length n works on in LDS: n itself for powers
+// of two and 2^a 3^b 5^c (Stockham passes), else the Bluestein length.
+int work_len(int n) { return is_pow2(n) || smooth235(n) ? n : next_pow2(2 * n - 1); }
+
+// Pass radices of a 2^a 3^b 5^c length: radix 8 while 8 divides what is
+// left of the power of two, then one pass of 4 or 2, then the 3s and 5s.
+std::vector<unsigned char> mixed_radices(int n)
+{
+	std::vector<unsigned char> r;
+	int p2 = 0;
+	while (n % 2 == 0) { n /= 2; ++p2; }
+	for (; p2 >= 3; p2 -= 3) r.push_back(8);
+	if (p2 == 2) r.push_back(4);
+	if (p2 == 1) r.push_back(2);
+	while (n % 3 == 0) { n /= 3; r.push_back(3); }
+	while (n % 5 == 0) { n /= 5; r.push_back(5); }
+	return r;
+}
 
 // Round-edge mask (GRoundEdge.cu:14-24) with centre (T/2, T/2), size (T, T).
 double round_edge(int x, int y, int tile)
@@ -353,7 +385,11 @@ int get_fft(gctf_ctx* c, int n, FftDev** out)
 	if (it != c->ffts.end()) { *out = it->second.get(); return GCTF_OK; }
 	auto f = std::make_unique<FftDev>();
 	f->n = n;
-	f->m = is_pow2(n) ? n : next_pow2(2 * n - 1);
+	f->m = work_len(n);
+	if (!is_pow2(n) && smooth235(n)) {
+		f->rad = mixed_radices(n);
+		if (f->rad.size() > sizeof(Fft1dPlan{}.rad)) return fail(c, GCTF_ERR_UNSUPPORTED, "transform length");
+	}
 	const int m = f->m;
 	if (m > 16384)
 		return fail(c, GCTF_ERR_UNSUPPORTED, "transform length " + std::to_string(n)
@@ -365,7 +401,7 @@ int get_fft(gctf_ctx* c, int n, FftDev** out)
 	}
 	CHK(ensure_buf(c, f->tw, sizeof(float2) * m));
 	HIPCHK(c, hipMemcpy(f->tw.p, tw.data(), sizeof(float2) * m, hipMemcpyHostToDevice));
-	if (!is_pow2(n)) {
+	if (!is_pow2(n) && f->rad.empty()) {
 		std::vector<std::complex<double>> w(n), b(m, 0.0);
 		std::vector<float2> chirp(n), bhat(m);
 		for (int k = 0; k < n; ++k) {
@@ -608,24 +644,29 @@ int make_rescale_plan(gctf_ctx* c, int nx, int ny, float pix, RescalePlan& rp)
 	rp.pitch = (rp.nnx / 2 + 1) * 2;
 	rp.active = bin > 1;
 	if (rp.active) {
-		// an even row length whose Bluestein working length exceeds one LDS
-		// transform runs as a half-length complex transform
-		auto half = [](int n) { return n % 2 == 0 && !is_pow2(n) && next_pow2(2 * n - 1) > 16384; };
+		// an even row length whose working length exceeds one LDS transform
+		// runs as a half-length complex transform
+		auto half = [](int n) { return n % 2 == 0 && !is_pow2(n) && work_len(n) > 16384; };
 		rp.rows_half = half(nx);
 		rp.c2r_half = half(rp.nnx);
-		// a row whose Bluestein length is 16384 also runs at half length, on
-		// 8192-point transforms: 73.7 KB of LDS, two workgroups per CU
-		auto half8 = [](int n) { return n % 2 == 0 && !is_pow2(n) && next_pow2(2 * n - 1) == 16384; };
+		// a row whose working length is above 8192 also runs at half length
+		// when that fits 8192-point transforms: 73.7 KB of LDS, two
+		// workgroups per CU
+		auto half8 = [](int n) {
+			return n % 2 == 0 && !is_pow2(n) && work_len(n) > 8192 && work_len(n) <= 16384 && work_len(n / 2) <= 8192;
+		};
 		if (half8(nx)) { rp.rows_half = true; rp.rows_maxn = 8192; }
 		if (half8(rp.nnx)) { rp.c2r_half = true; rp.c2r_maxn = 8192; }
 		rp.cols_split = half(ny);
 		if (rp.cols_split && rp.nny > 8192)
 			return fail(c, GCTF_ERR_UNSUPPORTED, "rescaled height " + std::to_string(rp.nny)
 				+ " above 8192 for a split column transform");
-		// a column whose Bluestein length is 16384 also splits when its
-		// rescaled height fits 8192-point transforms: 73.7 KB of LDS, two
+		// a column whose working length is above 8192 also splits when its
+		// halves fit 8192-point transforms and the rescaled height is at most
+		// 4096 (the split kernel keeps up to MAXN/2 rows): 73.7 KB of LDS, two
 		// workgroups per CU instead of one
-		if (!rp.cols_split && ny % 2 == 0 && !is_pow2(ny) && next_pow2(2 * ny - 1) == 16384 && rp.nny <= 4096) {
+		if (!rp.cols_split && ny % 2 == 0 && !is_pow2(ny) && work_len(ny) > 8192 && work_len(ny) <= 16384
+			&& work_len(ny / 2) <= 8192 && rp.nny <= 4096) {
 			rp.cols_split = true;
 			rp.cols_maxn = 8192;
 		}

@@ -3,9 +3,10 @@
 // Power-of-two lengths use a Stockham autosort (radix 8, last pass radix 2
 // or 4) whose butterflies live in registers; one workgroup-wide barrier
 // separates the LDS reads of a pass from its writes, so one LDS buffer of N
-// complex values suffices.  Other lengths use Bluestein's chirp-z algorithm
-// on a power-of-two length M >= 2n-1.  Both are unnormalised; the forward
-// transform uses exp(-2 pi i k n / N).
+// complex values suffices.  Lengths 2^a 3^b 5^c run the same Stockham passes
+// with radix-3 and radix-5 butterflies added (fft_lds_mixed).  Other lengths
+// use Bluestein's chirp-z algorithm on a power-of-two length M >= 2n-1.  All
+// are unnormalised; the forward transform uses exp(-2 pi i k n / N).
 //
 // LDS layout: logical element i lives at physical index pidx(i) = i + i/8.
 // The Stockham writes of the early passes stride by the radix (8 complex);
@@ -99,6 +100,49 @@ template <>
 __device__ __forceinline__ void dft_small<8, false>(float2* u) { dft8<false>(u); }
 template <>
 __device__ __forceinline__ void dft_small<8, true>(float2* u) { dft8<true>(u); }
+
+// u1 (+-) i s u2 parts of the radix-3 and radix-5 butterflies: y = a + mul_wi(b)
+// is a - i b forward, a + i b inverse.
+template <bool INV>
+__device__ __forceinline__ void dft3(float2* u)
+{
+	const float s3 = 0.86602540378443864676f;     // sin(2 pi / 3)
+	const float2 t1 = cadd(u[1], u[2]);
+	const float2 t2 = make_float2(fmaf(-0.5f, t1.x, u[0].x), fmaf(-0.5f, t1.y, u[0].y));
+	const float2 d = mul_wi<INV>(make_float2(s3 * (u[1].x - u[2].x), s3 * (u[1].y - u[2].y)));
+	u[0] = cadd(u[0], t1);
+	u[1] = cadd(t2, d);
+	u[2] = csub(t2, d);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft5(float2* u)
+{
+	const float c1 = 0.30901699437494742410f;     // cos(2 pi / 5)
+	const float c2 = -0.80901699437494742410f;    // cos(4 pi / 5)
+	const float s1 = 0.95105651629515357212f;     // sin(2 pi / 5)
+	const float s2 = 0.58778525229247312917f;     // sin(4 pi / 5)
+	const float2 a1 = cadd(u[1], u[4]), a2 = cadd(u[2], u[3]);
+	const float2 b1 = csub(u[1], u[4]), b2 = csub(u[2], u[3]);
+	const float2 p1 = make_float2(fmaf(c2, a2.x, fmaf(c1, a1.x, u[0].x)), fmaf(c2, a2.y, fmaf(c1, a1.y, u[0].y)));
+	const float2 p2 = make_float2(fmaf(c1, a2.x, fmaf(c2, a1.x, u[0].x)), fmaf(c1, a2.y, fmaf(c2, a1.y, u[0].y)));
+	const float2 q1 = mul_wi<INV>(make_float2(fmaf(s2, b2.x, s1 * b1.x), fmaf(s2, b2.y, s1 * b1.y)));
+	const float2 q2 = mul_wi<INV>(make_float2(fmaf(-s1, b2.x, s2 * b1.x), fmaf(-s1, b2.y, s2 * b1.y)));
+	u[0] = cadd(cadd(u[0], a1), a2);
+	u[1] = cadd(p1, q1);
+	u[4] = csub(p1, q1);
+	u[2] = cadd(p2, q2);
+	u[3] = csub(p2, q2);
+}
+
+template <>
+__device__ __forceinline__ void dft_small<3, false>(float2* u) { dft3<false>(u); }
+template <>
+__device__ __forceinline__ void dft_small<3, true>(float2* u) { dft3<true>(u); }
+template <>
+__device__ __forceinline__ void dft_small<5, false>(float2* u) { dft5<false>(u); }
+template <>
+__device__ __forceinline__ void dft_small<5, true>(float2* u) { dft5<true>(u); }
 
 // u[r] *= w^r for r = 1 .. R-1 (w already conjugated for the inverse).
 template <int R>
@@ -457,16 +501,80 @@ __device__ __forceinline__ void bluestein_lds(float2* buf, int n, int m, const f
 // A 1D transform of arbitrary length n in LDS, as planned on the host.
 struct Fft1dPlan {
 	int n;                   // transform length
-	int m;                   // power-of-two working length (n when n is a power of two)
+	int m;                   // working length: n for powers of two and for lengths 2^a 3^b 5^c,
+	                         // else the Bluestein length, a power of two >= 2n-1
 	const float2* tw;        // m-point twiddles
 	const float2* chirp;     // n entries (Bluestein only, else null)
 	const float2* bhat;      // m entries (Bluestein only, else null)
+	int nrad;                // mixed-radix passes (lengths 2^a 3^b 5^c that are not powers of two), else 0
+	unsigned char rad[20];   // their radices, in pass order: 8s, then a 4 or 2, then 3s, then 5s
 };
+
+// ---- mixed-radix FFT (lengths 2^a 3^b 5^c) --------------------------------
+// The Stockham pass of radix R after passes of product L, for any radices:
+// butterfly j (T = n / R of them) reads j + r T, multiplies element r by
+// w^r with w = exp(-+2 pi i k S / n), k = j mod L, S = n / (L R), and
+// writes (j - k) R + k + r L.
+template <int R, int NT, int MAXN, bool INV>
+__device__ __forceinline__ void fft_pass_mx(float2* buf, const float2* __restrict__ tw, int tid,
+	int L, int T, int S)
+{
+	constexpr int BPT = (MAXN / R + NT - 1) / NT;
+	const bool lpow2 = (L & (L - 1)) == 0;
+	float2 u[BPT][R];
+	int kk[BPT];
+#pragma unroll
+	for (int b = 0; b < BPT; ++b) {
+		const int j = tid + b * NT;
+		kk[b] = 0;
+		if (j < T) {
+			kk[b] = lpow2 ? (j & (L - 1)) : j % L;
+#pragma unroll
+			for (int r = 0; r < R; ++r) u[b][r] = buf[pidx(j + r * T)];
+			if (L > 1) {
+				float2 w = tw[kk[b] * S];
+				if (INV) w.y = -w.y;
+				twiddle_powers<R>(u[b], w);
+			}
+			dft_small<R, INV>(u[b]);
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int b = 0; b < BPT; ++b) {
+		const int j = tid + b * NT;
+		if (j < T) {
+			const int base = (j - kk[b]) * R + kk[b];
+#pragma unroll
+			for (int r = 0; r < R; ++r) buf[pidx(base + r * L)] = u[b][r];
+		}
+	}
+	__syncthreads();
+}
+
+// In-place transform of a mixed-radix plan (n <= MAXN); same contract as
+// fft_lds_rt.
+template <int NT, int MAXN, bool INV>
+__device__ __forceinline__ void fft_lds_mixed(float2* buf, const Fft1dPlan& p, int tid)
+{
+	int L = 1;
+	for (int i = 0; i < p.nrad; ++i) {
+		const int R = p.rad[i];
+		const int T = p.n / R, S = p.n / (L * R);
+		if (R == 8) fft_pass_mx<8, NT, MAXN, INV>(buf, p.tw, tid, L, T, S);
+		else if (R == 4) fft_pass_mx<4, NT, MAXN, INV>(buf, p.tw, tid, L, T, S);
+		else if (R == 2) fft_pass_mx<2, NT, MAXN, INV>(buf, p.tw, tid, L, T, S);
+		else if (R == 3) fft_pass_mx<3, NT, MAXN, INV>(buf, p.tw, tid, L, T, S);
+		else fft_pass_mx<5, NT, MAXN, INV>(buf, p.tw, tid, L, T, S);
+		L *= R;
+	}
+}
 
 template <int NT, int MAXN, bool INV>
 __device__ __forceinline__ void fft_any_lds(float2* buf, const Fft1dPlan& p, int tid)
 {
-	if (p.chirp == nullptr) fft_lds_rt<NT, MAXN, INV>(buf, p.n, p.tw, tid);
+	if (p.nrad > 0) fft_lds_mixed<NT, MAXN, INV>(buf, p, tid);
+	else if (p.chirp == nullptr) fft_lds_rt<NT, MAXN, INV>(buf, p.n, p.tw, tid);
 	else if (p.m == MAXN) bluestein_io<NT, MAXN, INV>(buf, p.n, p.chirp, p.bhat, p.tw, tid, LdsIn{buf});
 	else bluestein_lds<NT, MAXN, INV>(buf, p.n, p.m, p.chirp, p.bhat, p.tw, tid);
 }

@@ -40,8 +40,12 @@ def _img(n=1024, pix=1.2, seed=1, **kw):
     return synth.micrograph_np(n, n, pix=pix, seed=seed, **kw)
 
 
-@pytest.mark.parametrize("shape,pix", [((1024, 1024), 1.0), ((1000, 900), 0.8), ((768, 1024), 1.2)])
+@pytest.mark.parametrize("shape,pix", [((1024, 1024), 1.0), ((1000, 900), 0.8), ((768, 1024), 1.2),
+                                       ((720, 960), 0.6), ((1200, 1500), 0.8)])
 def test_rescale_matches_oracle(shape, pix):
+    """Includes lengths 2^a 3^b 5^c in every stage, which run as mixed-radix
+    Stockham transforms: 960 x 720 at 0.6 A crops to 480 x 360, and
+    1500 x 1200 at 0.8 A to 1000 x 800."""
     ny, nx = shape
     _check_rescale(_img(max(shape), pix)[:ny, :nx].copy(), pix)
 
@@ -51,23 +55,30 @@ def test_rescale_matches_oracle(shape, pix):
                                        ((256, 5760), 0.8), ((256, 5760), 1.0)])
 def test_rescale_long_rows_matches_oracle(shape, pix):
     """Rows of 11520 pixels (the super-resolution width): the R2C row transform
-    runs at half length, even and odd samples packed into one complex row; at
-    1.0 A the C2R rows (9600 pixels) do as well.  Columns of 11520 pixels
-    (portrait): the column pass splits its forward transform into those of
-    the even and odd samples; columns of 8184 pixels (the super-resolution
-    height) split too, onto 8192-point transforms, and so do rows of 5760
-    (R2C) and 4800 (C2R at 1.0 A) pixels."""
+    runs at half length, even and odd samples packed into one complex row, as
+    a 5760-point mixed-radix transform; at 1.0 A the C2R rows (9600 pixels)
+    do as well.  Columns of 11520 pixels (portrait): the column pass splits
+    its forward transform into those of the even and odd samples when the
+    rescaled height fits 8192-point transforms (0.5 A), else runs it whole on
+    the 16384-point kernel (0.7 A); columns of 8184 pixels (the
+    super-resolution height) split onto 8192-point Bluestein transforms.
+    Rows of 5760 pixels run as whole 5760-point mixed-radix transforms."""
     from gctffind_amd import synth
     _check_rescale(synth.micrograph_np(shape[0], shape[1], pix=pix, seed=7), pix)
 
 
 def test_rescale_split_columns_limit():
     """A split column transform whose rescaled height exceeds 8192 is refused
-    with an error, not computed."""
+    with an error, not computed.  11522 = 2 x 7 x 823 has no mixed-radix
+    plan: its columns split into two 5761-point Bluestein transforms, and at
+    1.0 A the rescaled height is 9602.  (11520 = 2^8 3^2 5 instead fits one
+    11520-point mixed-radix transform and needs no split.)"""
     _dev()
     from gctffind_amd import gctf
     with pytest.raises(RuntimeError, match="8192"):
-        _ctx().rescale_setup(256, 11520, 1.0)
+        _ctx().rescale_setup(256, 11522, 1.0)
+    nnx, nny, pitch, pixn = _ctx().rescale_setup(256, 11520, 1.0)
+    assert nny == 9600
 
 
 @pytest.mark.parametrize("nx", [11520, 5760])
