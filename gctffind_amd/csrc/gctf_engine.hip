@@ -2252,15 +2252,29 @@ int gctf_learned_program_default(gctf_search_program* out)
 namespace {
 
 struct SeriesCtl {
-	int ref;   // CFindSeriesCtfs's reference package: best score so far
+	int ref;    // CFindSeriesCtfs's reference package: best score so far
+	int next;   // packages order[0 .. next) are settled
+	int full;   // package that needs the full search this round, or -1
+	int pad;
 };
 
-__global__ void k_series_set_ref(SeriesCtl* ctl, int k) { ctl->ref = k; }
+// Packages in processing order (CLoadImages.cpp:71-91): the zero-tilt
+// package z first, then the others in file order.  The first package's full
+// search has run, so it is the reference and settled.
+__global__ void k_series_init(SeriesCtl* __restrict__ ctl, int* __restrict__ order, int n, int z)
+{
+	for (int j = threadIdx.x; j < n; j += blockDim.x) order[j] = j == 0 ? z : (j - 1 < z ? j - 1 : j);
+	if (threadIdx.x == 0) { ctl->ref = z; ctl->next = 1; ctl->full = -1; }
+}
 
 // CFindSeriesCtfs::mProcessRefine arguments (CFindSeriesCtfs.cpp:168-203)
-// from the reference package's results.
-__global__ void k_series_refine_prep(const SearchState* __restrict__ st, RefineArgs* __restrict__ refine,
-	const SeriesCtl* __restrict__ ctl, int k, float pix_in, int tomo, int phase_on)
+// from the reference package's results, for every package not yet settled;
+// settled packages sit the round out.  Every pending package refines around
+// the same reference: the results of those after the next change of
+// reference are discarded by k_series_walk and computed again.
+__global__ void k_series_round_prep(SearchState* __restrict__ st, RefineArgs* __restrict__ refine,
+	const SeriesCtl* __restrict__ ctl, const int* __restrict__ order, int n, float pix_in, int tomo,
+	int phase_on)
 {
 	const SearchState& R = st[ctl->ref];
 	RefineArgs r;
@@ -2278,34 +2292,57 @@ __global__ void k_series_refine_prep(const SearchState* __restrict__ st, RefineA
 		r.ang[1] = 30.0f;
 		if (phase_on) r.phase[1] = 40.0f;
 	}
-	refine[k] = r;
+	const int next = ctl->next;
+	__syncthreads();   // every thread has read the reference before any state changes
+	for (int j = threadIdx.x; j < n; j += blockDim.x) {
+		const int k = order[j];
+		if (j < next) {
+			st[k].skip = 1;
+		} else {
+			st[k].skip = 0;
+			refine[k] = r;
+		}
+	}
 }
 
-// After a refine: the better score becomes the reference (:200-202); a
-// score <= 0 sends the package to a full search (:144), which then runs
-// with CFindCtfBase::SetPhase's values; otherwise the full search sits out.
-__global__ void k_series_after_refine(SearchState* __restrict__ st, SeriesCtl* __restrict__ ctl, int k, float p0,
-	float range)
+// The sequential package rules over the round's refines, in processing
+// order: a better score makes the package the reference (:200-202), a score
+// <= 0 sends it to the full search (:144), which makes it the reference too
+// (:164).  Either ends the round after that package: the later ones refined
+// around a reference that is no longer current.  One thread.
+__global__ void k_series_walk(SearchState* __restrict__ st, SeriesCtl* __restrict__ ctl,
+	const int* __restrict__ order, int n, float p0, float range)
 {
-	SearchState& s = st[k];
-	if (st[ctl->ref].score < s.score) ctl->ref = k;
-	if (s.score <= 0.0f) {
-		s.skip = 0;
-		s.fallback = 1;
-		s.ext_phase = p0;
-		s.phase_range = range;
-	} else {
-		s.skip = 1;
-		s.fallback = 0;
+	int ref = ctl->ref, j = ctl->next, full = -1;
+	for (; j < n; ++j) {
+		const int k = order[j];
+		const SearchState& s = st[k];
+		bool stop = false;
+		if (st[ref].score < s.score) { ref = k; stop = true; }
+		if (s.score <= 0.0f) { full = k; stop = true; }
+		if (stop) { ++j; break; }
+	}
+	ctl->ref = ref;
+	ctl->next = j;
+	ctl->full = full;
+	// only the package that needs it takes part in the full search, with
+	// CFindCtfBase::SetPhase's values
+	for (int i = 0; i < n; ++i) {
+		SearchState& s = st[i];
+		s.skip = i == full ? 0 : 1;
+		if (i == full) {
+			s.fallback = 1;
+			s.ext_phase = p0;
+			s.phase_range = range;
+		}
 	}
 }
 
 // mProcessFull makes its package the reference (:164).
-__global__ void k_series_after_full(SearchState* __restrict__ st, SeriesCtl* __restrict__ ctl, int k)
+__global__ void k_series_after_full(SearchState* __restrict__ st, SeriesCtl* __restrict__ ctl, int n)
 {
-	SearchState& s = st[k];
-	if (!s.skip) ctl->ref = k;
-	s.skip = 0;
+	if (ctl->full >= 0) ctl->ref = ctl->full;
+	for (int i = 0; i < n; ++i) st[i].skip = 0;
 }
 
 // CInputFolder::FindZeroTilt (CInputFolder.cpp:49-68)
@@ -2353,43 +2390,55 @@ int gctf_estimate_series(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, 
 		CHK(ensure_rescale_buffers(ctx, rp, C));
 		CHK(ensure_buf(ctx, ctx->padc, sizeof(float) * (size_t)rp.pitch * rp.nny * C));
 	}
-	CHK(ensure_buf(ctx, ctx->series, sizeof(SeriesCtl)));
+	CHK(ensure_buf(ctx, ctx->series, sizeof(SeriesCtl) + sizeof(int) * (size_t)n));
 	// CLoadImages.cpp:71-91: the zero-tilt package first, the others in order
-	std::vector<int> order;
 	const int z = tomo ? zero_tilt(tilts_deg, n) : 0;
-	order.push_back(z);
-	for (int i = 0; i < n; ++i)
-		if (i != z) order.push_back(i);
 	const float p0 = ctx->prm.ext_phase_deg, range = std::max(ctx->prm.ext_phase_range_deg, 0.0f);
 	const int phase_on = ctx->prm.ext_phase_range_deg > 0.0f ? 1 : 0;
 	const float refine_phase = (!tomo && phase_on) ? 40.0f : 0.0f;
 	SeriesCtl* ctl = ctx->series.as<SeriesCtl>();
+	int* order = reinterpret_cast<int*>(ctl + 1);
 	SearchState* st = ctx->states.as<SearchState>();
-	auto record = [&]() -> int {
+	// the spectra, then mProcessFull for the first package (:155-166)
+	auto record_first = [&]() -> int {
 		CHK(record_spectra(ctx, rp, *tt, dev_imgs, dtype, n, stride));
 		CHK(fill_plans(ctx, 0, n, *sp));
-		const int k0 = order[0];
-		// mProcessFull for the first package (:155-166)
-		CHK(launch_1d(ctx, *tt, k0, 1, 0, 0.f, 0.f, ctx->prm.ext_phase_range_deg > 0.0f));
-		CHK(launch_2d(ctx, T, k0, 1, NX_BEGIN_DO2D, range));
-		k_series_set_ref<<<1, 1, 0, ctx->stream>>>(ctl, k0);
-		for (int j = 1; j < n; ++j) {
-			const int k = order[j];
-			k_series_refine_prep<<<1, 1, 0, ctx->stream>>>(st, ctx->refine.as<RefineArgs>(), ctl, k, pix, tomo,
-				phase_on);
-			CHK(launch_2d(ctx, T, k, 1, NX_BEGIN_REFINE, refine_phase));
-			k_series_after_refine<<<1, 1, 0, ctx->stream>>>(st, ctl, k, p0, range);
-			CHK(launch_1d(ctx, *tt, k, 1, 0, 0.f, 0.f, ctx->prm.ext_phase_range_deg > 0.0f));
-			CHK(launch_2d(ctx, T, k, 1, NX_BEGIN_DO2D, range));
-			k_series_after_full<<<1, 1, 0, ctx->stream>>>(st, ctl, k);
-		}
+		CHK(launch_1d(ctx, *tt, z, 1, 0, 0.f, 0.f, ctx->prm.ext_phase_range_deg > 0.0f));
+		CHK(launch_2d(ctx, T, z, 1, NX_BEGIN_DO2D, range));
+		k_series_init<<<1, 256, 0, ctx->stream>>>(ctl, order, n, z);
+		HIPCHK(ctx, hipGetLastError());
+		return GCTF_OK;
+	};
+	// one round: every pending package refines around the current reference
+	// in one batch, the package rules settle them in order up to the first
+	// change of reference, and a package whose refine failed gets its full
+	// search
+	auto record_round = [&]() -> int {
+		k_series_round_prep<<<1, 256, 0, ctx->stream>>>(st, ctx->refine.as<RefineArgs>(), ctl, order, n, pix,
+			tomo, phase_on);
+		CHK(launch_2d(ctx, T, 0, n, NX_BEGIN_REFINE, refine_phase));
+		k_series_walk<<<1, 1, 0, ctx->stream>>>(st, ctl, order, n, p0, range);
+		CHK(launch_1d(ctx, *tt, 0, n, 0, 0.f, 0.f, ctx->prm.ext_phase_range_deg > 0.0f));
+		CHK(launch_2d(ctx, T, 0, n, NX_BEGIN_DO2D, range));
+		k_series_after_full<<<1, 1, 0, ctx->stream>>>(st, ctl, n);
 		HIPCHK(ctx, hipGetLastError());
 		return GCTF_OK;
 	};
 	char key[192];
-	snprintf(key, sizeof key, "series/%p/%d/%d/%d/%d/%lld/%a/%d/%d/%d/%d/%d", dev_imgs, dtype, n, nx, ny, stride,
-		pix, C, ctx->tile_splits, ctx->cpb, tomo, z);
-	CHK(run_graph(ctx, key, record));
+	snprintf(key, sizeof key, "series/%p/%d/%d/%d/%d/%lld/%a/%d/%d/%d/%d/%d/%a/%a", dev_imgs, dtype, n, nx, ny,
+		stride, pix, C, ctx->tile_splits, ctx->cpb, tomo, z, p0, range);
+	CHK(run_graph(ctx, key, record_first));
+	char rkey[192];
+	snprintf(rkey, sizeof rkey, "series-round/%d/%a/%d/%d/%d/%a/%a", n, pix, ctx->cpb, tomo, phase_on, p0, range);
+	// typically one round: the zero-tilt package keeps the best score; each
+	// change of reference or full search costs one more, at most n - 1
+	for (int round = 1; round < n; ++round) {
+		CHK(run_graph(ctx, rkey, record_round));
+		SeriesCtl h{};
+		HIPCHK(ctx, hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+		HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+		if (h.next >= n) break;
+	}
 	return read_results(ctx, n, out);
 }
 

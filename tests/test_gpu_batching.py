@@ -104,6 +104,36 @@ def test_series_matches_oracle_package_loop(tomo):
         assert abs(got[i]["score"] - ref[i]["score"]) <= 1e-4, (i, got[i], ref[i])
 
 
+def test_series_reference_changes_match_oracle_package_loop():
+    """A tilt series whose later files score higher than the zero-tilt
+    package, so the reference moves (to files 4, 5 and 6 in processing
+    order): the batched refine rounds must settle each package against the
+    reference the sequential package loop would have used.  In tomo mode
+    astigmatism and azimuth stay at the reference's values, so a stale
+    reference would show in dfMin / dfMax."""
+    gctf = _setup()
+    from gctffind_amd import synth
+    pix = 1.35
+    tilts = np.array([-9.0, -6.0, -3.0, 0.0, 3.0, 6.0, 9.0], np.float32)
+    snr = [0.02, 0.03, 0.04, 0.08, 0.2, 0.4, 0.8]
+    imgs = []
+    for k, t in enumerate(tilts):
+        df = 20000.0 + 1500.0 * math.sin(math.radians(t))
+        ast = 150.0 + 60.0 * k
+        imgs.append(synth.micrograph_np(1024, 1024, pix=pix, seed=700 + k, df1=df + ast, df2=df - ast,
+                                        az_deg=20.0 + 7.0 * k, snr=snr[k]))
+    d = torch.from_numpy(np.stack(imgs)).cuda()
+    got = gctf.Context().estimate_series(d.data_ptr(), len(imgs), 1024, 1024, pix, tilts, tomo=True)
+    ref = oracle_series(imgs, pix, tilts)
+    for i in range(len(imgs)):
+        assert abs(got[i]["df_min"] - ref[i]["df_min"]) <= 5.0, (i, got[i], ref[i])
+        assert abs(got[i]["df_max"] - ref[i]["df_max"]) <= 5.0, (i, got[i], ref[i])
+        assert abs(got[i]["azimuth"] - ref[i]["azimuth"]) <= 0.5, (i, got[i], ref[i])
+        assert abs(got[i]["score"] - ref[i]["score"]) <= 1e-4, (i, got[i], ref[i])
+    # the scores rise after the zero-tilt package, so the reference did move
+    assert got[4]["score"] > got[3]["score"] and got[6]["score"] > got[5]["score"] > got[4]["score"]
+
+
 def _oracle_series_single(imgs, pix):
     """CFindSeriesCtfs for single-image packages in one series (not tomo):
     Refine also searches astigmatism (+-0.02) and azimuth (+-15 deg)."""
