@@ -2410,14 +2410,18 @@ int gctf_estimate_series(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, 
 		return GCTF_OK;
 	};
 	// one round: every pending package refines around the current reference
-	// in one batch, the package rules settle them in order up to the first
-	// change of reference, and a package whose refine failed gets its full
-	// search
+	// in one batch, and the package rules settle them in order up to the
+	// first change of reference or failed refine
 	auto record_round = [&]() -> int {
 		k_series_round_prep<<<1, 256, 0, ctx->stream>>>(st, ctx->refine.as<RefineArgs>(), ctl, order, n, pix,
 			tomo, phase_on);
 		CHK(launch_2d(ctx, T, 0, n, NX_BEGIN_REFINE, refine_phase));
 		k_series_walk<<<1, 1, 0, ctx->stream>>>(st, ctl, order, n, p0, range);
+		HIPCHK(ctx, hipGetLastError());
+		return GCTF_OK;
+	};
+	// the full search of the package whose refine failed (the others sit out)
+	auto record_full = [&]() -> int {
 		CHK(launch_1d(ctx, *tt, 0, n, 0, 0.f, 0.f, ctx->prm.ext_phase_range_deg > 0.0f));
 		CHK(launch_2d(ctx, T, 0, n, NX_BEGIN_DO2D, range));
 		k_series_after_full<<<1, 1, 0, ctx->stream>>>(st, ctl, n);
@@ -2428,8 +2432,9 @@ int gctf_estimate_series(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, 
 	snprintf(key, sizeof key, "series/%p/%d/%d/%d/%d/%lld/%a/%d/%d/%d/%d/%d/%a/%a", dev_imgs, dtype, n, nx, ny,
 		stride, pix, C, ctx->tile_splits, ctx->cpb, tomo, z, p0, range);
 	CHK(run_graph(ctx, key, record_first));
-	char rkey[192];
+	char rkey[192], fkey[192];
 	snprintf(rkey, sizeof rkey, "series-round/%d/%a/%d/%d/%d/%a/%a", n, pix, ctx->cpb, tomo, phase_on, p0, range);
+	snprintf(fkey, sizeof fkey, "series-full/%d/%a/%d/%a/%a", n, pix, ctx->cpb, p0, range);
 	// typically one round: the zero-tilt package keeps the best score; each
 	// change of reference or full search costs one more, at most n - 1
 	for (int round = 1; round < n; ++round) {
@@ -2437,8 +2442,12 @@ int gctf_estimate_series(gctf_ctx* ctx, const void* dev_imgs, int dtype, int n, 
 		SeriesCtl h{};
 		HIPCHK(ctx, hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
 		HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+		if (h.full >= 0) CHK(run_graph(ctx, fkey, record_full));
 		if (h.next >= n) break;
 	}
+	// no package sits out later calls on this context
+	k_clear_skip<<<(n + 63) / 64, 64, 0, ctx->stream>>>(st, n);
+	HIPCHK(ctx, hipGetLastError());
 	return read_results(ctx, n, out);
 }
 
