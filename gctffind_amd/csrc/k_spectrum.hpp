@@ -539,12 +539,28 @@ __device__ __forceinline__ void row_sums_body(const Tin* __restrict__ img, int p
 	const Tin* src = img + (long long)(g.offy + r) * pitch;
 	float* rs = rowsum + (long long)r * g.nbx;
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	for (int bi = wave; bi < g.nbx; bi += 4) {
-		const int x0 = g.offx + bi * g.step;
-		float a = 0.f;
-		for (int i = lane; i < g.bsize; i += 64) a += load_px(src, x0 + i);
-		a = wave_row_sum(a);
-		if (lane == 0) rs[bi] = a;
+	// wave w sums blocks w, w + 4, ...: the loads of PB of them are issued
+	// before any is reduced (each block's sum keeps its lane order and
+	// butterfly, so the bits do not depend on PB)
+	constexpr int PB = 4;
+	for (int b0 = wave; b0 < g.nbx; b0 += 4 * PB) {
+		float a[PB];
+#pragma unroll
+		for (int q = 0; q < PB; ++q) {
+			const int bi = b0 + 4 * q;
+			a[q] = 0.f;
+			if (bi < g.nbx) {
+				const int x0 = g.offx + bi * g.step;
+#pragma unroll 4
+				for (int i = lane; i < g.bsize; i += 64) a[q] += load_px(src, x0 + i);
+			}
+		}
+#pragma unroll
+		for (int q = 0; q < PB; ++q) {
+			const int bi = b0 + 4 * q;
+			const float t = wave_row_sum(a[q]);
+			if (bi < g.nbx && lane == 0) rs[bi] = t;
+		}
 	}
 }
 
