@@ -362,6 +362,41 @@ def test_ctf2d_axes_match_1d_and_generator():
     assert np.allclose(c2[:, :256], g[:, :256], atol=5e-4)
 
 
+def test_full_spectrum_layout_and_painting():
+    """orc_fc_full_spectrum, the -OutMrc diagnostic (CFindCtfBase.cpp:118-133,
+    CSpectrumImage.cpp:42-71, GCalcCTF2D.cu:49-78): the right half is the
+    half spectrum, the left half its point mirror, and inside the resolution
+    ring [25 p, max(2 p / 0.8, 3.5)] A the left half is painted with
+    (|CTF| - 1/2) 1.5 sd + mean of the half spectrum at the fitted
+    parameters.  The mirror is built with numpy flips, the CTF by the
+    synthetic generator's own atan2 formula, the moments by numpy."""
+    from gctffind_amd import synth
+    img = synth.micrograph_np(1024, 1024, pix=1.2, seed=3, df1=15500.0, df2=14500.0, az_deg=30.0)
+    nnx, nny, _, pixn, _ = O.rescale_sizes(1024, 1024, 1.2)
+    fc = O.FindCtf2D(pix=pixn)
+    fc.gen_half_spectrum(O.rescale(img, 1.2), nnx, nny)
+    r = fc.do2d()
+    out = fc.full_spectrum()
+    h = fc.spects()[1]
+    assert out.shape == (512, 512)
+    # right half: the half spectrum's columns 0..255 as they are
+    assert np.array_equal(out[:, 256:], h[:, :256])
+    # left half: the point mirror (y -> -y mod 512, x -> 256 - x)
+    mirror = np.roll(np.flip(h, axis=0), 1, axis=0)[:, 256:0:-1]
+    y, bx = np.mgrid[0:512, 0:256]
+    f = np.sqrt(((bx - 256) * 0.5 / 256.0) ** 2 + ((y - 256.0) / 512.0) ** 2)
+    p = np.float32(pixn)
+    ring = (f >= p / (25.0 * p)) & (f <= p / max(2.0 * p / 0.8, 3.5))
+    assert ring.sum() > 10000 and (~ring).sum() > 10000
+    assert np.array_equal(out[:, :256][~ring], mirror[~ring])
+    # the painted ring: |CTF| at the point-mirrored frequency (the CTF is even)
+    g, _ = synth.ctf_rfft(512, 512, pixn, df1=r["df_max"], df2=r["df_min"], az_deg=r["azimuth"],
+                          ext_phase_deg=r["ext_phase"])
+    g = np.abs(g.numpy())[(256 - y) % 512, 256 - bx]
+    want = (g - 0.5) * 1.5 * h.std() + h.mean()
+    assert np.allclose(out[:, :256][ring], want[ring], rtol=0, atol=1e-3 * 1.5 * h.std())
+
+
 # ---------------------------------------------------------------- a16/a19 CC
 def test_cc1d_brute_and_invariance():
     n = 257
