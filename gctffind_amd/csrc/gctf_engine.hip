@@ -95,6 +95,7 @@ struct FftDev {
 	int n = 0, m = 0;
 	std::vector<unsigned char> rad;   // mixed-radix passes (empty: power of two or Bluestein)
 	DevBuf tw, chirp, bhat;
+	DevBuf sub_chirp, sub_bhat, sub_tw, ct_tw;   // radix-4 step over quarter-length Bluestein transforms
 	Fft1dPlan plan() const
 	{
 		Fft1dPlan p{};
@@ -104,6 +105,10 @@ struct FftDev {
 		p.bhat = bhat.p ? bhat.as<float2>() : nullptr;
 		p.nrad = (int)rad.size();
 		for (size_t i = 0; i < rad.size(); ++i) p.rad[i] = rad[i];
+		p.sub_chirp = sub_chirp.p ? sub_chirp.as<float2>() : nullptr;
+		p.sub_bhat = sub_bhat.p ? sub_bhat.as<float2>() : nullptr;
+		p.sub_tw = sub_tw.p ? sub_tw.as<float2>() : nullptr;
+		p.ct_tw = ct_tw.p ? ct_tw.as<float2>() : nullptr;
 		return p;
 	}
 };
@@ -401,23 +406,46 @@ int get_fft(gctf_ctx* c, int n, FftDev** out)
 	}
 	CHK(ensure_buf(c, f->tw, sizeof(float2) * m));
 	HIPCHK(c, hipMemcpy(f->tw.p, tw.data(), sizeof(float2) * m, hipMemcpyHostToDevice));
-	if (!is_pow2(n) && f->rad.empty()) {
-		std::vector<std::complex<double>> w(n), b(m, 0.0);
-		std::vector<float2> chirp(n), bhat(m);
-		for (int k = 0; k < n; ++k) {
-			long long k2 = ((long long)k * k) % (2LL * n);
-			double a = -M_PI * (double)k2 / (double)n;
+	// Bluestein tables of length n on m points: chirp[k] = exp(-pi i (k^2 mod
+	// 2n) / n) and bhat = FFT_m of the wrapped conj(chirp)
+	auto bluestein_tables = [&](int len, int work, DevBuf& dchirp, DevBuf& dbhat) -> int {
+		std::vector<std::complex<double>> w(len), b(work, 0.0);
+		std::vector<float2> chirp(len), bhat(work);
+		for (int k = 0; k < len; ++k) {
+			long long k2 = ((long long)k * k) % (2LL * len);
+			double a = -M_PI * (double)k2 / (double)len;
 			w[k] = std::complex<double>(cos(a), sin(a));
 			chirp[k] = make_float2((float)w[k].real(), (float)w[k].imag());
 		}
 		b[0] = std::conj(w[0]);
-		for (int k = 1; k < n; ++k) { b[k] = std::conj(w[k]); b[m - k] = b[k]; }
+		for (int k = 1; k < len; ++k) { b[k] = std::conj(w[k]); b[work - k] = b[k]; }
 		host_fft(b, false);
-		for (int k = 0; k < m; ++k) bhat[k] = make_float2((float)b[k].real(), (float)b[k].imag());
-		CHK(ensure_buf(c, f->chirp, sizeof(float2) * n));
-		CHK(ensure_buf(c, f->bhat, sizeof(float2) * m));
-		HIPCHK(c, hipMemcpy(f->chirp.p, chirp.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
-		HIPCHK(c, hipMemcpy(f->bhat.p, bhat.data(), sizeof(float2) * m, hipMemcpyHostToDevice));
+		for (int k = 0; k < work; ++k) bhat[k] = make_float2((float)b[k].real(), (float)b[k].imag());
+		CHK(ensure_buf(c, dchirp, sizeof(float2) * len));
+		CHK(ensure_buf(c, dbhat, sizeof(float2) * work));
+		HIPCHK(c, hipMemcpy(dchirp.p, chirp.data(), sizeof(float2) * len, hipMemcpyHostToDevice));
+		HIPCHK(c, hipMemcpy(dbhat.p, bhat.data(), sizeof(float2) * work, hipMemcpyHostToDevice));
+		return GCTF_OK;
+	};
+	auto twiddles = [&](int len, DevBuf& d) -> int {
+		std::vector<float2> t(len);
+		for (int e = 0; e < len; ++e) {
+			double a = -2.0 * M_PI * (double)e / (double)len;
+			t[e] = make_float2((float)cos(a), (float)sin(a));
+		}
+		CHK(ensure_buf(c, d, sizeof(float2) * len));
+		HIPCHK(c, hipMemcpy(d.p, t.data(), sizeof(float2) * len, hipMemcpyHostToDevice));
+		return GCTF_OK;
+	};
+	if (!is_pow2(n) && f->rad.empty()) {
+		CHK(bluestein_tables(n, m, f->chirp, f->bhat));
+		// n = 4 q whose quarter has Bluestein length m / 4: also the tables
+		// of one radix-4 step over four q-point Bluestein transforms
+		if (n % 4 == 0 && next_pow2(2 * (n / 4) - 1) == m / 4) {
+			CHK(bluestein_tables(n / 4, m / 4, f->sub_chirp, f->sub_bhat));
+			CHK(twiddles(m / 4, f->sub_tw));
+			CHK(twiddles(n, f->ct_tw));
+		}
 	}
 	*out = f.get();
 	c->ffts[n] = std::move(f);

@@ -32,6 +32,12 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 	return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
 }
 
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b)
+{
+	return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -(a.x * b.y)));
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
@@ -508,6 +514,13 @@ struct Fft1dPlan {
 	const float2* bhat;      // m entries (Bluestein only, else null)
 	int nrad;                // mixed-radix passes (lengths 2^a 3^b 5^c that are not powers of two), else 0
 	unsigned char rad[20];   // their radices, in pass order: 8s, then a 4 or 2, then 3s, then 5s
+	// Bluestein lengths n = 4 q whose quarter q has Bluestein length m / 4:
+	// the transform may run as one radix-4 Cooley-Tukey step over four
+	// q-point Bluestein transforms (ct4_bluestein); else sub_chirp is null
+	const float2* sub_chirp; // q entries: the q-point chirp
+	const float2* sub_bhat;  // m / 4 entries
+	const float2* sub_tw;    // m / 4-point twiddles
+	const float2* ct_tw;     // n entries: exp(-2 pi i e / n)
 };
 
 // ---- mixed-radix FFT (lengths 2^a 3^b 5^c) --------------------------------
@@ -570,13 +583,74 @@ __device__ __forceinline__ void fft_lds_mixed(float2* buf, const Fft1dPlan& p, i
 	}
 }
 
+// ---- one radix-4 step over Bluestein transforms ---------------------------
+// For n = 4 q: X[k1 + q k2] = sum_g W^(g k1) Y_g[k1] exp(-+2 pi i g k2 / 4),
+// W = exp(-+2 pi i / n), with Y_g the q-point DFT of x[4 m + g].  Thread
+// group g (NT/4 threads) runs Y_g as a Bluestein transform on M/4 points in
+// its quarter of buf; the four read their inputs (src, which may read buf)
+// in the first pass, before its barrier, so no quarter is written before
+// every input is read.  The combine reads all Y into registers before the
+// barrier that precedes its writes.  Needs NT/4 > 64 (barriers span the
+// workgroup) and MAXN/4 = the plan's m/4.  Result in logical buf[0..n).
+template <int NT, int MAXN, bool INV, class Src>
+__device__ __forceinline__ void ct4_bluestein(float2* buf, const Fft1dPlan& p, int tid, const Src& src)
+{
+	constexpr int SNT = NT / 4, SM = MAXN / 4;
+	static_assert(SNT > 64, "the sub-transforms' barriers must span the workgroup");
+	const int g = tid / SNT, lt = tid - g * SNT;
+	const int q = p.n / 4;
+	float2* sb = buf + g * fft_lds_len(SM);           // = buf + pidx(g SM): SM is a multiple of 8
+	auto ld = [&](int i) -> float2 { return src(4 * i + g); };
+	bluestein_io<SNT, SM, INV>(sb, q, p.sub_chirp, p.sub_bhat, p.sub_tw, lt, ld);
+	constexpr int PER = (MAXN / 4 + NT - 1) / NT;
+	float2 y[PER][4];
+#pragma unroll
+	for (int b = 0; b < PER; ++b) {
+		const int t = tid + b * NT;
+		if (t < q) {
+#pragma unroll
+			for (int h = 0; h < 4; ++h) {
+				const float2 v = buf[h * fft_lds_len(SM) + pidx(t)];
+				if (h == 0) {
+					y[b][h] = v;
+				} else {
+					const float2 w = p.ct_tw[h * t];
+					y[b][h] = INV ? cmulc(v, w) : cmul(v, w);
+				}
+			}
+			dft4<INV>(y[b][0], y[b][1], y[b][2], y[b][3]);
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int b = 0; b < PER; ++b) {
+		const int t = tid + b * NT;
+		if (t < q) {
+#pragma unroll
+			for (int h = 0; h < 4; ++h) buf[pidx(t + h * q)] = y[b][h];
+		}
+	}
+	__syncthreads();
+}
+
 template <int NT, int MAXN, bool INV>
 __device__ __forceinline__ void fft_any_lds(float2* buf, const Fft1dPlan& p, int tid)
 {
-	if (p.nrad > 0) fft_lds_mixed<NT, MAXN, INV>(buf, p, tid);
-	else if (p.chirp == nullptr) fft_lds_rt<NT, MAXN, INV>(buf, p.n, p.tw, tid);
-	else if (p.m == MAXN) bluestein_io<NT, MAXN, INV>(buf, p.n, p.chirp, p.bhat, p.tw, tid, LdsIn{buf});
-	else bluestein_lds<NT, MAXN, INV>(buf, p.n, p.m, p.chirp, p.bhat, p.tw, tid);
+	if (p.nrad > 0) {
+		fft_lds_mixed<NT, MAXN, INV>(buf, p, tid);
+	} else if (p.chirp == nullptr) {
+		fft_lds_rt<NT, MAXN, INV>(buf, p.n, p.tw, tid);
+	} else if (p.m == MAXN) {
+		if constexpr (NT / 4 > 64) {
+			if (p.sub_chirp != nullptr) {
+				ct4_bluestein<NT, MAXN, INV>(buf, p, tid, LdsIn{buf});
+				return;
+			}
+		}
+		bluestein_io<NT, MAXN, INV>(buf, p.n, p.chirp, p.bhat, p.tw, tid, LdsIn{buf});
+	} else {
+		bluestein_lds<NT, MAXN, INV>(buf, p.n, p.m, p.chirp, p.bhat, p.tw, tid);
+	}
 }
 
 // The same transform with the input element i < p.n given by src(i) instead
@@ -586,6 +660,12 @@ template <int NT, int MAXN, bool INV, class Src>
 __device__ __forceinline__ void fft_any_lds_src(float2* buf, const Fft1dPlan& p, int tid, const Src& src)
 {
 	if (p.chirp != nullptr && p.m == MAXN) {
+		if constexpr (NT / 4 > 64) {
+			if (p.sub_chirp != nullptr) {
+				ct4_bluestein<NT, MAXN, INV>(buf, p, tid, src);
+				return;
+			}
+		}
 		bluestein_io<NT, MAXN, INV>(buf, p.n, p.chirp, p.bhat, p.tw, tid, src);
 		return;
 	}

@@ -41,11 +41,16 @@ def _img(n=1024, pix=1.2, seed=1, **kw):
 
 
 @pytest.mark.parametrize("shape,pix", [((1024, 1024), 1.0), ((1000, 900), 0.8), ((768, 1024), 1.2),
-                                       ((720, 960), 0.6), ((1200, 1500), 0.8)])
+                                       ((720, 960), 0.6), ((1200, 1500), 0.8),
+                                       ((4096, 256), 1.0), ((256, 4096), 1.0), ((4092, 256), 0.8)])
 def test_rescale_matches_oracle(shape, pix):
     """Includes lengths 2^a 3^b 5^c in every stage, which run as mixed-radix
     Stockham transforms: 960 x 720 at 0.6 A crops to 480 x 360, and
-    1500 x 1200 at 0.8 A to 1000 x 800."""
+    1500 x 1200 at 0.8 A to 1000 x 800.  And lengths n = 4 q whose quarter
+    has a quarter of n's Bluestein length, which run as one radix-4 step
+    over four q-point Bluestein transforms: the 3412-point inverse columns
+    (4096 high at 1.0 A) and C2R rows (4096 wide at 1.0 A) of the 4096^2
+    workload, and 4092-point forward and 2728-point inverse columns."""
     ny, nx = shape
     _check_rescale(_img(max(shape), pix)[:ny, :nx].copy(), pix)
 
