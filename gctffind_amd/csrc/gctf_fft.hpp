@@ -183,10 +183,75 @@ __device__ __forceinline__ void fft_sync()
 	}
 }
 
+// Twiddle sources of the power-of-two passes.  Pass P (its L > 1) of
+// butterfly b of thread j uses w = exp(-2 pi i k S / N) with k = j mod L,
+// S = N / (L R).  TwTable loads it from the table where the pass needs it;
+// TwRegs holds every pass's values in registers, loaded once (fft_tw_load)
+// by a kernel that runs many transforms of one length with the same threads:
+// the passes then wait for no memory, and a load issued before the
+// transform (the next transform's input) stays in flight through it.
+struct TwTable {
+	const float2* __restrict__ tw;
+	template <int N, int R, int L, int P>
+	__device__ __forceinline__ float2 get(int, int j) const { return tw[(j & (L - 1)) * (N / (L * R))]; }
+};
+
+// Passes of the power-of-two Stockham transform of length N (radix 8, the
+// last 2 or 4) and the largest butterfly count per thread of any of them.
+__host__ __device__ constexpr int fft_npass(int n)
+{
+	int p = 0;
+	for (int l = 1; l < n; l *= (n / l >= 8 ? 8 : n / l)) ++p;
+	return p;
+}
+__host__ __device__ constexpr int fft_max_bpt(int n, int nt)
+{
+	int m = 1;
+	for (int l = 1; l < n; l *= (n / l >= 8 ? 8 : n / l)) {
+		const int t = n / (n / l >= 8 ? 8 : n / l);
+		m = (t + nt - 1) / nt > m ? (t + nt - 1) / nt : m;
+	}
+	return m;
+}
+
+template <int N, int NT>
+struct TwRegs {
+	float2 v[fft_npass(N)][fft_max_bpt(N, NT)];
+	template <int N2, int R, int L, int P>
+	__device__ __forceinline__ float2 get(int b, int) const { return v[P][b]; }
+};
+
+template <int N, int L, int NT, int P>
+__device__ __forceinline__ void fft_tw_load_run(TwRegs<N, NT>& w, const float2* __restrict__ tw, int tid)
+{
+	if constexpr (L < N) {
+		constexpr int rem = N / L;
+		constexpr int R = rem >= 8 ? 8 : rem;
+		if constexpr (L > 1) {
+			constexpr int T = N / R;
+			constexpr int BPT = (T + NT - 1) / NT;
+#pragma unroll
+			for (int b = 0; b < BPT; ++b) {
+				const int j = tid + b * NT;
+				if (T % NT == 0 || j < T) w.v[P][b] = tw[(j & (L - 1)) * (N / (L * R))];
+			}
+		}
+		fft_tw_load_run<N, L * R, NT, P + 1>(w, tw, tid);
+	}
+}
+
+// The twiddles thread tid of fft_lds<N, NT, INV> uses, from the table tw.
+template <int N, int NT>
+__device__ __forceinline__ void fft_tw_load(TwRegs<N, NT>& w, const float2* __restrict__ tw, int tid)
+{
+	fft_tw_load_run<N, 1, NT, 0>(w, tw, tid);
+}
+
 // One Stockham pass of radix R over N points; L = product of the radices of
-// the previous passes.  NT threads (tid in [0, NT)) share the buffer.
-template <int N, int R, int L, int NT, bool INV>
-__device__ __forceinline__ void fft_pass(float2* buf, const float2* __restrict__ tw, int tid)
+// the previous passes; P its index.  NT threads (tid in [0, NT)) share the
+// buffer.
+template <int N, int R, int L, int P, int NT, bool INV, class TW>
+__device__ __forceinline__ void fft_pass(float2* buf, const TW& twf, int tid)
 {
 	constexpr int T = N / R;
 	constexpr int BPT = (T + NT - 1) / NT;
@@ -198,9 +263,7 @@ __device__ __forceinline__ void fft_pass(float2* buf, const float2* __restrict__
 #pragma unroll
 			for (int r = 0; r < R; ++r) u[b][r] = buf[pidx(j + r * T)];
 			if constexpr (L > 1) {
-				const int k = j & (L - 1);
-				constexpr int S = N / (L * R);
-				float2 w = tw[k * S];
+				float2 w = twf.template get<N, R, L, P>(b, j);
 				if (INV) w.y = -w.y;
 				twiddle_powers<R>(u[b], w);
 			}
@@ -221,14 +284,14 @@ __device__ __forceinline__ void fft_pass(float2* buf, const float2* __restrict__
 	fft_sync<NT>();
 }
 
-template <int N, int L, int NT, bool INV>
-__device__ __forceinline__ void fft_run(float2* buf, const float2* __restrict__ tw, int tid)
+template <int N, int L, int P, int NT, bool INV, class TW>
+__device__ __forceinline__ void fft_run(float2* buf, const TW& twf, int tid)
 {
 	if constexpr (L < N) {
 		constexpr int rem = N / L;
 		constexpr int R = rem >= 8 ? 8 : rem;
-		fft_pass<N, R, L, NT, INV>(buf, tw, tid);
-		fft_run<N, L * R, NT, INV>(buf, tw, tid);
+		fft_pass<N, R, L, P, NT, INV>(buf, twf, tid);
+		fft_run<N, L * R, P + 1, NT, INV>(buf, twf, tid);
 	}
 }
 
@@ -241,7 +304,14 @@ __device__ __forceinline__ void fft_run(float2* buf, const float2* __restrict__ 
 template <int N, int NT, bool INV>
 __device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw, int tid)
 {
-	fft_run<N, 1, NT, INV>(buf, tw, tid);
+	fft_run<N, 1, 0, NT, INV>(buf, TwTable{tw}, tid);
+}
+
+// The same with the twiddles already in registers (fft_tw_load).
+template <int N, int NT, bool INV>
+__device__ __forceinline__ void fft_lds(float2* buf, const TwRegs<N, NT>& w, int tid)
+{
+	fft_run<N, 1, 0, NT, INV>(buf, w, tid);
 }
 
 // ---- compile-time transform with input and output hooks -----------------

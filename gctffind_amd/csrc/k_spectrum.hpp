@@ -659,8 +659,8 @@ __device__ __forceinline__ void tile_rows_body(const Tin* __restrict__ im, int p
 	const int nblk = g.ntx * g.nty * NRB;
 	const int grp = threadIdx.x / FT, lane = threadIdx.x % FT;
 	float2* fb = buf + grp * FL;
-	// this thread's pixels of the two rows of row block b
-	float va[PX], vb[PX];
+	// this thread's pixels of the two rows of row block b, and their mask
+	float va[PX], vb[PX], ma[PX], mb[PX];
 	auto load_rows = [&](int b) {
 		const int t = b / NRB, ya = (b % NRB) * ROWS + 2 * grp;
 		const int sx = g.offx + (t % g.ntx) * g.step, sy = g.offy + (t / g.ntx) * g.step;
@@ -671,12 +671,27 @@ __device__ __forceinline__ void tile_rows_body(const Tin* __restrict__ im, int p
 			vb[i] = load_px(src, (long long)pitch + lane + i * FT);
 		}
 	};
+	auto load_mask = [&](int b) {
+		const int ya = (b % NRB) * ROWS + 2 * grp;
+#pragma unroll
+		for (int i = 0; i < PX; ++i) {
+			ma[i] = mask[ya * T + lane + i * FT];
+			mb[i] = mask[(ya + 1) * T + lane + i * FT];
+		}
+	};
 	const int b0 = item * G;
 	if (b0 >= nblk) return;
 	load_rows(b0);
+	load_mask(b0);
+	// Loads complete in issue order, and a wait after a branch that may
+	// have issued loads counts them as issued: the next block's loads are
+	// therefore unconditional (clamped to the last block) and issued after
+	// the loads the transform waits for, so they stay in flight through it.
+#pragma unroll
 	for (int it = 0; it < G; ++it) {
 		const int b = b0 + it;
 		if (b >= nblk) break;
+		const int nb = min(b + 1, nblk - 1);
 		const int rb = b % NRB, t = b / NRB;
 		const int tx = t % g.ntx, ty = t / g.ntx;
 		float tsum;
@@ -687,15 +702,16 @@ __device__ __forceinline__ void tile_rows_body(const Tin* __restrict__ im, int p
 			tsum = block_sum(bs, ty * g.nbx + tx);
 		}
 		const float mean = tsum / ((float)T * (float)T);
-		const int ya = rb * ROWS + 2 * grp;
 #pragma unroll
 		for (int i = 0; i < PX; ++i) {
 			const int x = lane + i * FT;
-			fb[pidx(x)] = make_float2((va[i] - mean) * mask[ya * T + x], (vb[i] - mean) * mask[(ya + 1) * T + x]);
+			fb[pidx(x)] = make_float2((va[i] - mean) * ma[i], (vb[i] - mean) * mb[i]);
 		}
 		fft_sync<FT>();
-		if (it + 1 < G && b + 1 < nblk) load_rows(b + 1);
-		fft_lds<T, FT, false>(fb, tw, lane);
+		TwRegs<T, FT> twr;
+		fft_tw_load(twr, tw, lane);
+		if (it + 1 < G) load_rows(nb);
+		fft_lds<T, FT, false>(fb, twr, lane);
 		float2 sa[KP], sb[KP];
 #pragma unroll
 		for (int i = 0; i < KP; ++i) {
@@ -722,6 +738,7 @@ __device__ __forceinline__ void tile_rows_body(const Tin* __restrict__ im, int p
 			int k = i / ROWS, yy = i % ROWS;
 			dst[(long long)k * T + yy] = stage[k * SP + yy];
 		}
+		if (it + 1 < G) load_mask(nb);
 		// the stage is the transforms' buffer: read out before it is refilled
 		__syncthreads();
 	}
@@ -796,13 +813,15 @@ __device__ __forceinline__ void tile_cols_body(const float2* __restrict__ src, i
 #pragma unroll
 	for (int i = 0; i < PER; ++i) acc[i] = 0.f;
 	float2* fb = buf + grp * FL;
+	TwRegs<T, FT> twr;
+	fft_tw_load(twr, tw, lane);
 	for (int t = t0; t < t1; ++t) {
 		if (live) {
 			const float2* col = src + ((long long)t * CX + kx) * T;
 			for (int y = lane; y < T; y += FT) fb[pidx(y)] = col[y];
 		}
 		fft_sync<FT>();
-		fft_lds<T, FT, false>(fb, tw, lane);
+		fft_lds<T, FT, false>(fb, twr, lane);
 #pragma unroll
 		for (int i = 0; i < PER; ++i) {
 			float2 c = fb[pidx(lane + i * FT)];
