@@ -49,8 +49,8 @@ VALU_PEAK_TFLOPS = 78.6   # FP32 FMA without packed math: 256 CU x 64 lanes x 2 
 # scaled by the same counters on a 256 MiB device copy (fetch x2.000,
 # write x1.000); the table is profiles/pmc_traffic_plain_batch32_4096sq.md.
 MEASURED_TRAFFIC_PER_IMAGE = {
-    "k_rescale_rows_r2c": 123.08e6, "k_transpose_c": 138.41e6, "k_rescale_cols": 104.36e6,
-    "k_rescale_rows_c2r": 93.70e6, "k_tile_rows": 196.62e6, "k_tile_cols": 165.29e6,
+    "k_rescale_rows_r2c": 123.08e6, "k_transpose_c": 138.40e6, "k_rescale_cols": 104.03e6,
+    "k_rescale_rows_c2r": 93.65e6, "k_tile_rows": 196.53e6, "k_tile_cols": 165.02e6,
 }
 
 # Wave64 VALU instructions per 4096x4096 micrograph at 1.0 A (rocprofv3 --pmc
@@ -59,7 +59,7 @@ MEASURED_TRAFFIC_PER_IMAGE = {
 # measured against: one wave64 FP32 instruction per CU per cycle (a
 # microbenchmark of independent FMA/add/multiply chains reaches 0.97).
 MEASURED_VALU_INSTR_PER_IMAGE = {
-    "k_rescale_rows_r2c": 9.380e6, "k_rescale_cols": 40.531e6, "k_rescale_rows_c2r": 35.797e6,
+    "k_rescale_rows_r2c": 9.380e6, "k_rescale_cols": 36.724e6, "k_rescale_rows_c2r": 32.100e6,
 }
 VALU_ISSUE_PEAK = 256 * 2.4e9          # wave64 instructions per second
 
@@ -70,8 +70,8 @@ SPECTRUM_KERNELS = ("k_rescale_rows_r2c", "k_transpose_c", "k_rescale_cols", "k_
                     "k_row_sums", "k_block_rows", "k_tile_rows", "k_tile_cols", "k_tile_reduce",
                     "k_bg_rows", "k_bg_cols", "k_gate", "k_lp_x", "k_lp_y")
 MEASURED_TRAFFIC_SMALL_PER_IMAGE = {
-    "k_block_rows": 0.25e6, "k_tile_reduce": 2.63e6, "k_bg_rows": 1.57e6, "k_bg_cols": 5.29e6,
-    "k_gate": 0.01e6, "k_lp_x": 1.61e6, "k_lp_y": 7.18e6,
+    "k_block_rows": 0.24e6, "k_tile_reduce": 2.63e6, "k_bg_rows": 1.57e6, "k_bg_cols": 5.29e6,
+    "k_gate": 0.01e6, "k_lp_x": 1.61e6, "k_lp_y": 7.21e6,
 }
 
 
@@ -87,11 +87,13 @@ def spectrum_bytes_per_image(nx, ny, in_bytes, nnx, nny, rescaled, tile=512):
 
 
 # Arithmetic of the FFT kernels per micrograph (5 N log2 N per complex
-# transform of length N, 6 per complex multiply): k_rescale_cols runs 1707
-# columns of FFT 4096 + Bluestein 3412 (two FFTs of 8192, three pointwise
-# products of 8192).
+# transform of length N, 6 per complex multiply, 16 per radix-4 butterfly):
+# k_rescale_cols runs 1707 columns of FFT 4096 + an inverse of 3412 = 4 x 853
+# as four 853-point Bluestein transforms (each two FFTs of 2048 and three
+# pointwise products of 2048) joined by 853 radix-4 butterflies with three
+# twiddle products each.
 FFT_FLOP_PER_IMAGE = {
-    "k_rescale_cols": 1707 * (5 * 4096 * 12 + 2 * 5 * 8192 * 13 + 3 * 6 * 8192),
+    "k_rescale_cols": 1707 * (5 * 4096 * 12 + 4 * (2 * 5 * 2048 * 11 + 3 * 6 * 2048) + 853 * (3 * 6 + 16)),
 }
 
 
@@ -601,7 +603,8 @@ def roofline(ctx, ptr, n, a, run, tuned=None):
         tf = fl * per / (rows[dom]["avg_us"] * 1e-6) / 1e12
         pk["valu"] = {"achieved": round(tf, 1), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                       "frac": round(tf / VALU_PEAK_TFLOPS, 4), "flop_per_launch": int(fl * per),
-                      "note": "FFT-bound: 5 N log2 N per complex transform"}
+                      "note": "FFT-bound: 5 N log2 N per complex transform of the transforms run (FFT 4096; "
+                              "four 853-point Bluestein transforms on 2048 joined by radix-4 butterflies)"}
     vi = MEASURED_VALU_INSTR_PER_IMAGE.get(base) if std else None
     if vi:
         rate = vi * per / (rows[dom]["avg_us"] * 1e-6)
