@@ -282,6 +282,13 @@ struct gctf_ctx {
 	RescalePlan rp;
 	bool rp_ready = false, rp_done = false;
 	DevBuf pad1;
+	DevBuf hostin;   // a host image's device copy for gctf_rescale, kept between calls
+	// gctf_rescale_prefetch: the next host image's copy, made on copy_stream
+	DevBuf prefetch;
+	const float* prefetch_src = nullptr;
+	size_t prefetch_bytes = 0;
+	hipStream_t copy_stream = nullptr;
+	hipEvent_t prefetch_done = nullptr, main_done = nullptr;
 	float s_kv = 0, s_cs = 0, s_ac = 0, s_pix = 0;
 	bool setup1 = false, setup2 = false, have_raw = false, have_spect = false;
 	int s_nx = 0, s_ny = 0;
@@ -324,6 +331,9 @@ struct gctf_ctx {
 		net_buffers_destroy(netb);
 		for (auto& r : prec) { evpool.push_back(r.a); evpool.push_back(r.b); }
 		for (auto e : evpool) (void)hipEventDestroy(e);
+		if (prefetch_done) (void)hipEventDestroy(prefetch_done);
+		if (main_done) (void)hipEventDestroy(main_done);
+		if (copy_stream) (void)hipStreamDestroy(copy_stream);
 		if (own_stream) (void)hipStreamDestroy(own_stream);
 	}
 };
@@ -1503,6 +1513,26 @@ int gctf_rescale_setup(gctf_ctx* ctx, int nx, int ny, float raw_pix, int* new_si
 	return GCTF_OK;
 }
 
+// The device copy gctf_rescale_prefetch made of host image img (bytes long),
+// if the last prefetch was of that image: the context's stream waits for it,
+// and it becomes the staging buffer.  Null otherwise.  A prefetch serves
+// only the next rescale of a host image and is dropped by any other.
+static const float* take_prefetch(gctf_ctx* ctx, const float* img, size_t bytes, int* rc)
+{
+	*rc = GCTF_OK;
+	const bool match = ctx->prefetch_src == img && ctx->prefetch_bytes == bytes;
+	ctx->prefetch_src = nullptr;
+	if (!match) return nullptr;
+	const hipError_t e = hipStreamWaitEvent(ctx->stream, ctx->prefetch_done, 0);
+	if (e != hipSuccess) {
+		*rc = fail(ctx, GCTF_ERR_HIP, std::string("hipStreamWaitEvent: ") + hipGetErrorString(e));
+		return nullptr;
+	}
+	std::swap(ctx->hostin.p, ctx->prefetch.p);
+	std::swap(ctx->hostin.bytes, ctx->prefetch.bytes);
+	return ctx->hostin.as<float>();
+}
+
 int gctf_rescale(gctf_ctx* ctx, const float* img, int img_on_device)
 {
 	if (!ctx || !img) return fail(ctx, GCTF_ERR_ARG, "null image");
@@ -1510,26 +1540,59 @@ int gctf_rescale(gctf_ctx* ctx, const float* img, int img_on_device)
 	HIPCHK(ctx, hipSetDevice(ctx->device));
 	const RescalePlan& rp = ctx->rp;
 	float* out = ctx->pad1.as<float>();
+	const size_t bytes = sizeof(float) * (size_t)rp.nx * rp.ny;
+	int rc = GCTF_OK;
+	const float* pre = img_on_device ? nullptr : take_prefetch(ctx, img, bytes, &rc);
+	if (img_on_device) ctx->prefetch_src = nullptr;
+	CHK(rc);
 	if (!rp.active) {
 		// CRescaleImage.cpp:76-85: rows copied into the padded buffer
-		HIPCHK(ctx, hipMemcpy2DAsync(out, sizeof(float) * rp.pitch, img, sizeof(float) * rp.nx,
-			sizeof(float) * rp.nx, rp.nny, img_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-			ctx->stream));
+		const bool dev = img_on_device || pre;
+		HIPCHK(ctx, hipMemcpy2DAsync(out, sizeof(float) * rp.pitch, pre ? pre : img, sizeof(float) * rp.nx,
+			sizeof(float) * rp.nx, rp.nny, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
 	} else {
 		const float* dimg = img;
-		DevBuf tmp;
-		if (!img_on_device) {
-			CHK(ensure_buf(ctx, tmp, sizeof(float) * (size_t)rp.nx * rp.ny));
-			HIPCHK(ctx, hipMemcpyAsync(tmp.p, img, sizeof(float) * (size_t)rp.nx * rp.ny,
-				hipMemcpyHostToDevice, ctx->stream));
-			dimg = tmp.as<float>();
+		if (pre) {
+			dimg = pre;
+		} else if (!img_on_device) {
+			// the staging buffer stays with the context: no allocation, and no
+			// synchronising free, per image (no graph reads it, so growing it
+			// leaves the cached graphs valid)
+			const hipError_t e = ctx->hostin.ensure(bytes, nullptr);
+			if (e != hipSuccess)
+				return fail(ctx, GCTF_ERR_NOMEM, std::string("device allocation of ") + std::to_string(bytes)
+					+ " bytes failed: " + hipGetErrorString(e));
+			HIPCHK(ctx, hipMemcpyAsync(ctx->hostin.p, img, bytes, hipMemcpyHostToDevice, ctx->stream));
+			dimg = ctx->hostin.as<float>();
 		}
 		CHK(launch_rescale(ctx, rp, dimg, GCTF_DTYPE_F32, (long long)rp.nx * rp.ny, 1, out,
 			(long long)rp.pitch * rp.nny));
-		HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
 	}
 	HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
 	ctx->rp_done = true;
+	return GCTF_OK;
+}
+
+int gctf_rescale_prefetch(gctf_ctx* ctx, const float* img, int nx, int ny)
+{
+	if (!ctx || !img || nx < 1 || ny < 1) return fail(ctx, GCTF_ERR_ARG, "null image or empty size");
+	HIPCHK(ctx, hipSetDevice(ctx->device));
+	if (!ctx->copy_stream) HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+	if (!ctx->prefetch_done) HIPCHK(ctx, hipEventCreateWithFlags(&ctx->prefetch_done, hipEventDisableTiming));
+	if (!ctx->main_done) HIPCHK(ctx, hipEventCreateWithFlags(&ctx->main_done, hipEventDisableTiming));
+	const size_t bytes = sizeof(float) * (size_t)nx * ny;
+	ctx->prefetch_src = nullptr;
+	// the buffer may last have been read by work on the context's stream
+	HIPCHK(ctx, hipEventRecord(ctx->main_done, ctx->stream));
+	HIPCHK(ctx, hipStreamWaitEvent(ctx->copy_stream, ctx->main_done, 0));
+	const hipError_t e = ctx->prefetch.ensure(bytes, nullptr);
+	if (e != hipSuccess)
+		return fail(ctx, GCTF_ERR_NOMEM, std::string("device allocation of ") + std::to_string(bytes)
+			+ " bytes failed: " + hipGetErrorString(e));
+	HIPCHK(ctx, hipMemcpyAsync(ctx->prefetch.p, img, bytes, hipMemcpyHostToDevice, ctx->copy_stream));
+	HIPCHK(ctx, hipEventRecord(ctx->prefetch_done, ctx->copy_stream));
+	ctx->prefetch_src = img;
+	ctx->prefetch_bytes = bytes;
 	return GCTF_OK;
 }
 

@@ -66,6 +66,11 @@ void CRescaleImage::DoIt(float* pfImage)
 	require(gctf_rescale(SharedContext(), pfImage, 0), "CRescaleImage::DoIt");
 }
 
+void CRescaleImage::Prefetch(float* pfImage, int* piSize)
+{
+	require(gctf_rescale_prefetch(SharedContext(), pfImage, piSize[0], piSize[1]), "CRescaleImage::Prefetch");
+}
+
 float* CRescaleImage::GetScaledImg(void)
 {
 	float* p = nullptr;

@@ -46,6 +46,11 @@ public:
 	void Setup(int* piRawSize, float fRawPixSize);
 	// DoIt(float* pfImage): host image of the Setup size
 	void DoIt(float* pfImage);
+	// no reference counterpart: starts the device copy of the host image
+	// that the next DoIt will rescale (piSize its nx, ny), so that the
+	// transfer overlaps work in between; the image must stay unchanged
+	// until that DoIt returns
+	void Prefetch(float* pfImage, int* piSize);
 	// device pointer, rows of m_aiPadSizeN[0] floats; do not free
 	float* GetScaledImg(void);
 	int m_aiNewSize[2] = {0, 0};

@@ -253,6 +253,14 @@ public:
 		cv_.wait(l, [&] { return ready_[k] != 0; });
 		return items_[k];
 	}
+	// The k-th micrograph if it has been read, without waiting.
+	bool peek(size_t k, Item* out)
+	{
+		std::lock_guard<std::mutex> l(m_);
+		if (k >= ready_.size() || !ready_[k]) return false;
+		*out = items_[k];
+		return true;
+	}
 	// Returns the buffer of the k-th micrograph (taken with take(k)); no
 	// copy from it may be pending.
 	void give_back(size_t k, const Item& it)
@@ -557,6 +565,13 @@ int main(int argc, char** argv)
 			// CFindSeriesCtfs::mRescaleImage + mSetupFindCtf (:108-131)
 			rescale.Setup(size, o.pix);
 			rescale.DoIt(im.px);
+			// the next micrograph's transfer to the GPU, if it has been read,
+			// overlaps this one's estimation
+			ImageLoader::Item next;
+			if (k + 1 < order.size() && loader.peek(k + 1, &next) && next.ok) {
+				int nsize[2] = {next.nx, next.ny};
+				rescale.Prefetch(next.px, nsize);
+			}
 			CCTFTheory theory;
 			theory.Setup(o.kv, o.cs, o.ac, rescale.m_fPixSizeN, 100.0f, o.ext_phase[0] * 0.017453f);
 			find.Setup1(&theory);

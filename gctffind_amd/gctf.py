@@ -23,7 +23,7 @@ DTYPE_F32, DTYPE_F16 = 0, 1
 EXPORTS = [
     "gctf_create", "gctf_destroy", "gctf_error_string", "gctf_set_stream", "gctf_synchronize",
     "gctf_host_alloc", "gctf_host_free",
-    "gctf_rescale_setup", "gctf_rescale", "gctf_scaled_image", "gctf_setup1", "gctf_setup2",
+    "gctf_rescale_setup", "gctf_rescale", "gctf_rescale_prefetch", "gctf_scaled_image", "gctf_setup1", "gctf_setup2",
     "gctf_set_phase", "gctf_set_search1d", "gctf_gen_half_spectrum", "gctf_do1d", "gctf_refine1d", "gctf_do2d",
     "gctf_do2d_batch_static",
     "gctf_refine", "gctf_spect_size", "gctf_get_half_spect", "gctf_set_half_spect",
@@ -118,6 +118,7 @@ def load():
         "gctf_host_free": (None, [vp]),
         "gctf_rescale_setup": (ci, [vp, ci, ci, cf, fp, fp]),
         "gctf_rescale": (ci, [vp, fp, ci]),
+        "gctf_rescale_prefetch": (ci, [vp, fp, ci, ci]),
         "gctf_scaled_image": (ci, [vp, ctypes.POINTER(vp)]),
         "gctf_setup1": (ci, [vp, cf, cf, cf, cf, cf, cf]),
         "gctf_setup2": (ci, [vp, ci, ci]),
@@ -218,6 +219,15 @@ class Context:
         else:
             img = np.ascontiguousarray(img, np.float32)
             self.check(self.L.gctf_rescale(self.h, _np_ptr(img), 0))
+
+    def rescale_prefetch(self, img):
+        """Starts the device copy of host image img, which the next rescale
+        of the same array uses.  img must be a C-contiguous float32 array
+        that the caller keeps, unchanged, until that rescale returns."""
+        if not (isinstance(img, np.ndarray) and img.dtype == np.float32 and img.flags["C_CONTIGUOUS"]
+                and img.ndim == 2):
+            raise GctfError(GCTF_ERR_ARG, "prefetch needs a 2-D C-contiguous float32 array")
+        self.check(self.L.gctf_rescale_prefetch(self.h, _np_ptr(img), img.shape[1], img.shape[0]))
 
     def scaled_image(self):
         p = ctypes.c_void_p()

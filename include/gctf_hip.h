@@ -83,6 +83,15 @@ GCTF_API int gctf_rescale_setup(gctf_ctx* ctx, int nx, int ny, float raw_pix, in
 /* CRescaleImage::DoIt(float* pfImage) (:429): nx*ny float32 image, host
  * memory when img_on_device == 0, else device memory. */
 GCTF_API int gctf_rescale(gctf_ctx* ctx, const float* img, int img_on_device);
+/* No reference counterpart: starts copying the nx*ny float32 host image img
+ * to the device on a copy stream of the context and returns at once.  The
+ * next gctf_rescale, if it is of the same pointer and size, uses that copy
+ * instead of copying again; any other drops it.  For callers that estimate
+ * one micrograph at a time (the command line): the next micrograph's PCIe
+ * transfer then overlaps the current one's estimation.  img (page-locked
+ * memory, e.g. from gctf_host_alloc, for an asynchronous copy) must stay
+ * valid and unchanged until that gctf_rescale returns. */
+GCTF_API int gctf_rescale_prefetch(gctf_ctx* ctx, const float* img, int nx, int ny);
 /* CRescaleImage::GetScaledImg() (:430): device pointer, padded rows. */
 GCTF_API int gctf_scaled_image(gctf_ctx* ctx, float** dev_pad_img);
 

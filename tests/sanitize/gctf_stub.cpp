@@ -27,6 +27,10 @@ struct gctf_ctx {
 	float p0 = 0.0f, prange = 0.0f;
 	gctf_result last{};
 	std::string err;
+	// gctf_rescale_prefetch: the image and its checksum when the copy began
+	const float* pre_img = nullptr;
+	size_t pre_n = 0;
+	double pre_sum = 0.0;
 };
 
 extern "C" {
@@ -76,11 +80,35 @@ int gctf_rescale_setup(gctf_ctx* ctx, int nx, int ny, float raw_pix, int* new_si
 	return GCTF_OK;
 }
 
+// The real call copies the whole image asynchronously; the stand-in reads
+// it all now, and the next gctf_rescale of the same image checks that it did
+// not change in between (the caller's contract).
+int gctf_rescale_prefetch(gctf_ctx* ctx, const float* img, int nx, int ny)
+{
+	if (!ctx || !img || nx < 1 || ny < 1) return GCTF_ERR_ARG;
+	double s = 0.0;
+	for (size_t i = 0; i < (size_t)nx * ny; ++i) s += img[i];
+	ctx->pre_img = img;
+	ctx->pre_n = (size_t)nx * ny;
+	ctx->pre_sum = s;
+	return GCTF_OK;
+}
+
 // Every pixel of the caller's nx x ny image is read; the padded image holds
 // a nearest-neighbour copy of it.
 int gctf_rescale(gctf_ctx* ctx, const float* img, int img_on_device)
 {
 	if (!ctx || !img || img_on_device) return GCTF_ERR_ARG;
+	if (ctx->pre_img == img) {
+		double c = 0.0;
+		for (size_t i = 0; i < ctx->pre_n; ++i) c += img[i];
+		ctx->pre_img = nullptr;
+		if (c != ctx->pre_sum) {
+			ctx->err = "prefetched image changed before its rescale";
+			return GCTF_ERR_STATE;
+		}
+	}
+	ctx->pre_img = nullptr;
 	double s = 0.0;
 	for (size_t i = 0; i < (size_t)ctx->nx * ctx->ny; ++i) s += img[i];
 	ctx->mean = s / ((double)ctx->nx * ctx->ny);

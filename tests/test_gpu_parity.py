@@ -127,6 +127,44 @@ def _check_rescale(img, pix):
     assert np.array_equal(host2[:, :nnx], host[:, :nnx])
 
 
+@pytest.mark.parametrize("pix", [0.8, 1.2])
+def test_rescale_prefetch_uses_the_copy_and_drops_a_stale_one(pix):
+    """gctf_rescale_prefetch: the next rescale of the prefetched host image
+    gives the same bits as an ordinary one; a rescale of another image drops
+    the prefetch and copies its own (at 0.8 A through the rescale, at 1.2 A
+    through the unrescaled row copy)."""
+    _dev()
+    a = _img(1024, pix, seed=11)[:, :1000].copy()
+    b = _img(1024, pix, seed=12)[:, :1000].copy()
+    ny, nx = a.shape
+    ctx = _ctx()
+    nnx, nny, pitch, _ = ctx.rescale_setup(nx, ny, pix)
+
+    def scaled():
+        h = np.empty((nny, pitch), np.float32)
+        _copy_dev_to_host(ctx.scaled_image(), h)
+        return h[:, :nnx].copy()
+
+    ctx.rescale(a)
+    ra = scaled()
+    ctx.rescale(b)
+    rb = scaled()
+    assert not np.array_equal(ra, rb)
+    for _ in range(2):
+        ctx.rescale_prefetch(a)
+        ctx.rescale(a)
+        assert np.array_equal(scaled(), ra)
+        ctx.rescale_prefetch(b)
+        ctx.rescale(b)
+        assert np.array_equal(scaled(), rb)
+    # a prefetch of b, then a rescale of a: a's own copy, and b's is dropped
+    ctx.rescale_prefetch(b)
+    ctx.rescale(a)
+    assert np.array_equal(scaled(), ra)
+    ctx.rescale(b)
+    assert np.array_equal(scaled(), rb)
+
+
 def _copy_dev_to_host(ptr, host):
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
