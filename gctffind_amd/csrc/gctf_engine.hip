@@ -797,6 +797,16 @@ int ensure_image_buffers(gctf_ctx* c, int n, int T, int ring_n)
 // launch sequences
 // ---------------------------------------------------------------------------
 
+// Complex elements of one micrograph's part of each rescale buffer: the row
+// spectra (ny rows, spec_ld(ncx) apart) or the rescaled columns (nny x ncx),
+// rounded up to 256 bytes so that every micrograph's rows start on a line.
+size_t cbuf_elems(const RescalePlan& rp)
+{
+	const int ncx = rp.nnx / 2 + 1;
+	const size_t e = std::max((size_t)rp.ny * spec_ld(ncx), (size_t)ncx * rp.nny);
+	return (e + 31) / 32 * 32;
+}
+
 // Rescale n images (CRescaleImage::DoIt) into padded rows at out + i * out_stride.
 // With tg, the C2R pass also leaves the row partials of tg's mean blocks in
 // c->rowsum (launch_spectrum then reads them instead of the image).
@@ -806,12 +816,10 @@ int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long lon
 {
 	hipStream_t s = c->stream;
 	const int ncx = rp.nnx / 2 + 1;
-	const long long sa = (long long)rp.ny * ncx;       // rows x cropped columns
-	const long long sb = (long long)ncx * rp.nny;
 	float2* A = c->cbufA.as<float2>();
 	float2* B = c->cbufB.as<float2>();
-	const long long stA = std::max(sa, sb), stB = std::max(sa, sb);
-	// 1. rows R2C with x crop into A: A[y][kx]
+	const long long stA = (long long)cbuf_elems(rp), stB = stA;
+	// 1. rows R2C with x crop into A: A[y][kx], rows spec_ld(ncx) apart
 	{
 		const int m = rp.rows->m;
 		dim3 grid((rp.ny + 1) / 2, n);
@@ -881,8 +889,7 @@ int launch_rescale_t(gctf_ctx* c, const RescalePlan& rp, const Tin* in, long lon
 
 int ensure_rescale_buffers(gctf_ctx* c, const RescalePlan& rp, int n)
 {
-	const size_t ncx = rp.nnx / 2 + 1;
-	const size_t per = std::max((size_t)rp.ny * ncx, ncx * (size_t)rp.nny);
+	const size_t per = cbuf_elems(rp);
 	CHK(ensure_buf(c, c->cbufA, sizeof(float2) * per * n));
 	CHK(ensure_buf(c, c->cbufB, sizeof(float2) * per * n));
 	return GCTF_OK;
@@ -2799,8 +2806,7 @@ int gctf_estimate_list(gctf_ctx* ctx, const gctf_image* imgs, int n, gctf_result
 		J.part_off = (long long)G.part; G.part += (size_t)J.nsplit * T * (T / 2 + 1);
 		if (rp.active) {
 			pad_off[(size_t)i] = G.pad; G.pad += (size_t)rp.pitch * rp.nny;
-			const size_t ncx = rp.nnx / 2 + 1;
-			cbuf_off[(size_t)i] = G.cbuf; G.cbuf += std::max((size_t)rp.ny * ncx, ncx * (size_t)rp.nny);
+			cbuf_off[(size_t)i] = G.cbuf; G.cbuf += cbuf_elems(rp);
 		}
 		G.items += (ntiles * NRB + 1) / 2;
 		if (J.need_rows) G.max_nr = std::max(G.max_nr, nr);

@@ -25,6 +25,12 @@ __device__ __forceinline__ int xcd_group(int b, int n)
 	return i < n ? i : -1;
 }
 
+// Row stride (complex elements) of the row spectra the R2C pass writes for
+// the transpose: ncx rounded up to 16, so that every row starts on a
+// 128-byte line and the transpose's 32-element tile rows (256 bytes) read
+// two whole lines each.
+__host__ __device__ constexpr int spec_ld(int ncx) { return (ncx + 15) & ~15; }
+
 struct TileGrid {
 	int ntx, nty, offx, offy, step, tile;
 	int blocks;          // 1: tile sums from 2x2 blocks of step x step
@@ -73,8 +79,9 @@ __device__ __forceinline__ float wave_row_sum(float v)
 // MAXN = 8192.
 __host__ __device__ constexpr int rescale_waves(int maxn) { return maxn <= 8192 ? 8 : 4; }
 
-// Real rows -> half-spectrum rows kept for kx < ncx_out.  Two rows share one
-// complex transform (z = a + i b).  Row pair bx of one image.
+// Real rows -> half-spectrum rows kept for kx < ncx_out, spec_ld(ncx_out)
+// apart.  Two rows share one complex transform (z = a + i b).  Row pair bx
+// of one image.
 template <typename Tin, int NT, int MAXN>
 __device__ __forceinline__ void rows_r2c_body(const Tin* __restrict__ img, int nx, int ny, const Fft1dPlan& plan,
 	int ncx_out, float2* __restrict__ out, int bx)
@@ -91,7 +98,8 @@ __device__ __forceinline__ void rows_r2c_body(const Tin* __restrict__ img, int n
 	}
 	__syncthreads();
 	fft_any_lds<NT, MAXN, false>(buf, plan, tid);
-	float2* dst = out + (long long)y0 * ncx_out;
+	const int ld = spec_ld(ncx_out);
+	float2* dst = out + (long long)y0 * ld;
 	for (int k = tid; k < ncx_out; k += NT) {
 		float2 z = buf[pidx(k)];
 		float2 zc = buf[pidx(k == 0 ? 0 : nx - k)];
@@ -99,7 +107,7 @@ __device__ __forceinline__ void rows_r2c_body(const Tin* __restrict__ img, int n
 		float2 A = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
 		float2 B = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
 		dst[k] = A;
-		if (two) dst[ncx_out + k] = B;
+		if (two) dst[ld + k] = B;
 	}
 }
 
@@ -118,7 +126,8 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_r2c(co
 // transform of length h = nx/2 of z[m] = x[2m] + i x[2m+1].  With
 // E = (Z[k] + conj Z[h-k]) / 2 and O = -i (Z[k] - conj Z[h-k]) / 2 the
 // transforms of the even and odd samples, X[k] = E[k] + exp(-2 pi i k/nx) O[k]
-// for k < ncx_out <= h.  plan is the length-h plan.  Row y of one image.
+// for k < ncx_out <= h, rows spec_ld(ncx_out) apart.  plan is the length-h
+// plan.  Row y of one image.
 template <typename Tin, int NT, int MAXN>
 __device__ __forceinline__ void rows_r2c_half_body(const Tin* __restrict__ img, int nx, const Fft1dPlan& plan,
 	int ncx_out, float2* __restrict__ out, int y)
@@ -130,7 +139,7 @@ __device__ __forceinline__ void rows_r2c_half_body(const Tin* __restrict__ img, 
 	for (int m = tid; m < h; m += NT) buf[pidx(m)] = make_float2(load_px(src, 2 * m), load_px(src, 2 * m + 1));
 	__syncthreads();
 	fft_any_lds<NT, MAXN, false>(buf, plan, tid);
-	float2* dst = out + (long long)y * ncx_out;
+	float2* dst = out + (long long)y * spec_ld(ncx_out);
 	for (int k = tid; k < ncx_out; k += NT) {
 		const float2 z = buf[pidx(k)];
 		const float2 zc = buf[pidx(k == 0 ? 0 : h - k)];
@@ -153,17 +162,19 @@ __global__ __launch_bounds__(NT, rescale_waves(MAXN)) void k_rescale_rows_r2c_ha
 		out + blockIdx.y * out_stride, blockIdx.x);
 }
 
-// Complex transpose out[x][y] = in[y][x]; in is rows x cols.  32x32 tiles;
-// tile (bx, by) of one image.
+// Complex transpose out[x][y] = in[y][x]; in is rows x cols, rows
+// spec_ld(cols) apart (the R2C pass's output).  32x32 tiles; tile (bx, by)
+// of one image.
 __device__ __forceinline__ void transpose_body(const float2* __restrict__ src, int rows, int cols,
 	float2* __restrict__ dst, int bxi, int byi)
 {
+	const int ld = spec_ld(cols);
 	__shared__ float2 t[32][33];
 	const int bx = bxi * 32, by = byi * 32;
 	const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // ty in [0, 8)
 	for (int j = ty; j < 32; j += 8) {
 		int y = by + j, x = bx + tx;
-		if (y < rows && x < cols) t[j][tx] = src[(long long)y * cols + x];
+		if (y < rows && x < cols) t[j][tx] = src[(long long)y * ld + x];
 	}
 	__syncthreads();
 	for (int j = ty; j < 32; j += 8) {
@@ -437,7 +448,7 @@ struct RescaleJob {
 	int nnx, nny, ncx;        // rescaled size, kept columns nnx/2 + 1
 	int pitch;                // padded output row (floats)
 	Fft1dPlan rows, cf, ci, c2r;
-	float2* A;                // ny x ncx row spectra, then nny x ncx rescaled columns
+	float2* A;                // ny row spectra spec_ld(ncx) apart, then nny x ncx rescaled columns
 	float2* B;                // ncx x ny transposed spectrum
 	float* out;               // padded rescaled image
 	float* rowsum;            // mean-block row partials of g (null: none)
