@@ -49,8 +49,8 @@ VALU_PEAK_TFLOPS = 78.6   # FP32 FMA without packed math: 256 CU x 64 lanes x 2 
 # scaled by the same counters on a 256 MiB device copy (fetch x2.000,
 # write x1.000); the table is profiles/pmc_traffic_plain_batch32_4096sq.md.
 MEASURED_TRAFFIC_PER_IMAGE = {
-    "k_rescale_rows_r2c": 123.08e6, "k_transpose_c": 138.40e6, "k_rescale_cols": 104.03e6,
-    "k_rescale_rows_c2r": 93.65e6, "k_tile_rows": 196.53e6, "k_tile_cols": 165.02e6,
+    "k_rescale_rows_r2c": 123.08e6, "k_transpose_c": 112.03e6, "k_rescale_cols": 104.02e6,
+    "k_rescale_rows_c2r": 93.65e6, "k_tile_rows": 196.52e6, "k_tile_cols": 165.41e6,
 }
 
 # Wave64 VALU instructions per 4096x4096 micrograph at 1.0 A (rocprofv3 --pmc
