@@ -99,13 +99,16 @@ def test_tilt_series_cli_matches_oracle(tmp_path):
     assert spec.shape == (5, 512, 512) and np.isfinite(spec).all()
 
 
-def test_serial_folder_cli(tmp_path):
+@pytest.mark.parametrize("pix", [1.35, 0.9])
+def test_serial_folder_cli(tmp_path, pix):
     """-Serial 1: every micrograph of a folder is estimated (read ahead on the
     loader thread, float32 and float16 files) and its full spectrum written to
-    a file of its own (on the writer thread)."""
+    a file of its own (on the writer thread).  Each micrograph after the first
+    reaches the GPU through the copy started while the one before it was
+    estimated (gctf_rescale_prefetch): at 1.35 A into the unrescaled image,
+    at 0.9 A into the Fourier-crop rescale."""
     _gpu()
     from gctffind_amd import synth
-    pix = 1.35
     truth = [15000.0, 16500.0, 18000.0]
     for k, df in enumerate(truth):
         img = synth.micrograph_np(1024, 1024, pix=pix, seed=320 + k, df1=df + 250.0, df2=df - 250.0,
