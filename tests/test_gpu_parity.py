@@ -55,9 +55,9 @@ def test_rescale_matches_oracle(shape, pix):
     _check_rescale(_img(max(shape), pix)[:ny, :nx].copy(), pix)
 
 
-@pytest.mark.parametrize("shape,pix", [((256, 11520), 0.5), ((256, 11520), 1.0), ((11520, 256), 0.5),
-                                       ((11520, 256), 0.7), ((8184, 256), 0.5), ((8184, 256), 1.0),
-                                       ((256, 5760), 0.8), ((256, 5760), 1.0)])
+@pytest.mark.parametrize("shape,pix", [((256, 11520), 0.5), ((256, 11520), 1.0), ((256, 11520), 1.1),
+                                       ((11520, 256), 0.5), ((11520, 256), 0.7), ((8184, 256), 0.5),
+                                       ((8184, 256), 1.0), ((256, 5760), 0.8), ((256, 5760), 1.0)])
 def test_rescale_long_rows_matches_oracle(shape, pix):
     """Rows of 11520 pixels (the super-resolution width): the R2C row transform
     runs at half length, even and odd samples packed into one complex row, as
@@ -67,7 +67,10 @@ def test_rescale_long_rows_matches_oracle(shape, pix):
     rescaled height fits 8192-point transforms (0.5 A), else runs it whole on
     the 16384-point kernel (0.7 A); columns of 8184 pixels (the
     super-resolution height) split onto 8192-point Bluestein transforms.
-    Rows of 5760 pixels run as whole 5760-point mixed-radix transforms."""
+    Rows of 5760 pixels run as whole 5760-point mixed-radix transforms.  At
+    1.1 A the 11520-pixel rows rescale to 10560, whose half-length C2R rows of
+    5280 = 4 x 1320 run as four 1320-point Bluestein transforms on 4096 inside
+    the 16384-point kernel."""
     from gctffind_amd import synth
     _check_rescale(synth.micrograph_np(shape[0], shape[1], pix=pix, seed=7), pix)
 
