@@ -154,6 +154,11 @@ float* CFindCtfBase::GenFullSpectrum(void)
 	return out;
 }
 
+bool CFindCtfBase::GenFullSpectrum(float* pfOut)
+{
+	return pfOut && mCheck(gctf_full_spectrum(SharedContext(), pfOut), "GenFullSpectrum");
+}
+
 void CFindCtfBase::ShowResult(void)
 {
 	printf("%9.2f  %9.2f  %6.2f  %6.2f  %8.5f\n\n", m_fDfMin, m_fDfMax, m_fAstAng, m_fExtPhase, m_fScore);

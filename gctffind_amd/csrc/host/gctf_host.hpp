@@ -69,6 +69,9 @@ public:
 	void GetSpectSize(int* piSize, bool bHalf);
 	void GenHalfSpectrum(float* gfPadImg);                // device image
 	float* GenFullSpectrum(void);                         // new[], freed by caller
+	// no reference counterpart: the same into the caller's buffer of
+	// GetSpectSize(.., false) floats (e.g. page-locked memory, reused)
+	bool GenFullSpectrum(float* pfOut);
 	void ShowResult(void);
 	float m_fDfMin = 0.0f;
 	float m_fDfMax = 0.0f;

@@ -374,6 +374,22 @@ public:
 		}
 		cv_.notify_one();
 	}
+	// A buffer of n floats for a job: one the writer has finished with when
+	// there is one, so that no spectrum costs a fresh allocation and its
+	// page faults.
+	std::vector<float> buffer(size_t n)
+	{
+		std::vector<float> v;
+		{
+			std::lock_guard<std::mutex> l(m_);
+			if (!free_.empty()) {
+				v = std::move(free_.back());
+				free_.pop_back();
+			}
+		}
+		v.resize(n);
+		return v;
+	}
 	// Writes what is queued, then closes the stack file.
 	void finish()
 	{
@@ -409,10 +425,13 @@ private:
 				MrcWriter w;
 				if (w.Open(j.file, j.nx, j.ny, 1, 1.0f)) { w.WriteSection(0, j.px.data()); w.Close(); }
 			}
+			std::lock_guard<std::mutex> l(m_);
+			if (free_.size() < 8) free_.push_back(std::move(j.px));
 		}
 		if (stack) stack->Close();
 	}
 	std::deque<Job> q_;
+	std::vector<std::vector<float>> free_;   // buffers of written jobs, for reuse
 	bool done_ = false;
 	std::mutex m_;
 	std::condition_variable cv_;
@@ -545,6 +564,13 @@ int main(int argc, char** argv)
 
 	std::unique_ptr<SpectrumWriter> writer;
 	if (!o.out_mrc.empty()) writer = std::make_unique<SpectrumWriter>();
+	// full spectra come back from the GPU through page-locked memory kept for
+	// the run
+	struct PinnedBuf {
+		float* p = nullptr;
+		size_t cap = 0;
+		~PinnedBuf() { if (p) gctf_host_free(p); }
+	} fs_host;
 	ImageLoader loader(pk, order, 4, 4);
 	CRescaleImage rescale;
 	CFindCtf2D find;
@@ -625,15 +651,26 @@ int main(int argc, char** argv)
 			base_name(p.file).c_str(), done + 1, n - 1 - done, p.idx + 1, p.df_min, p.df_max, p.azimuth,
 			p.ext_phase, p.score);
 		if (writer) {
-			float* fs = find.GenFullSpectrum();
 			int ss[2];
 			find.GetSpectSize(ss, false);
-			if (fs) {
+			const size_t ns = (size_t)ss[0] * ss[1];
+			// a full-rate copy from the GPU, then into a recycled buffer of
+			// the writer
+			if (fs_host.cap < ns) {
+				if (fs_host.p) gctf_host_free(fs_host.p);
+				void* hp = nullptr;
+				fs_host.p = gctf_host_alloc(ns * sizeof(float), &hp) == GCTF_OK ? static_cast<float*>(hp) : nullptr;
+				fs_host.cap = fs_host.p ? ns : 0;
+			}
+			std::vector<float> fsv;
+			float* fs = fs_host.p;
+			if (!fs) { fsv.resize(ns); fs = fsv.data(); }
+			if (find.GenFullSpectrum(fs)) {
 				SpectrumWriter::Job j;
 				j.nx = ss[0];
 				j.ny = ss[1];
-				j.px.assign(fs, fs + (size_t)ss[0] * ss[1]);
-				delete[] fs;
+				j.px = writer->buffer(ns);
+				std::memcpy(j.px.data(), fs, ns * sizeof(float));
 				if (tomo || o.serial == 0) {
 					j.stack = true;
 					j.file = o.out_mrc;
